@@ -1,0 +1,203 @@
+/*
+ * mfs.h -- C ABI of libmfs_hip.so, the MI355X (gfx950) implementation of the
+ * MiniFS per-iteration solver path: Roe inviscid flux with residual
+ * assembly, local time step, LU-SGS forward/backward sweeps with the
+ * flow-field update, and node-based Green-Gauss gradients.
+ *
+ * Reference interfaces replaced (paths relative to the MiniFS tree):
+ *   ZeroResiduals(PolyGrid*)                src/Kernels/cal_flux.h:24
+ *   InviscidFlux(PolyGrid*, limit, level)   src/Kernels/cal_flux.h:27
+ *   InviscidFlux_TDTree(...)                src/Kernels/cal_flux.h:41
+ *   ComputeTimeStep(PolyGrid*)              src/Kernels/compute_timestep.h:24
+ *   ForwardLUSGS(PolyGrid*, level)          src/Kernels/forward_lusgs.h:23
+ *   CalculateGradient(PolyGrid*)            src/Kernels/cal_gradient.h:23
+ *   PassInterfaceData(PolyGrid*)            src/Kernels/load_grid.h:39
+ *   LoadGrid / Decoupling / PreprocessGrid  src/Kernels/load_grid.h:27,43,33
+ *   driver loop                             src/main.cpp:198-291
+ *   solver globals                          src/Base/para_field_global.h:32-273
+ *
+ * Conventions.  Every function returns an int status (MFS_OK = 0, negative
+ * on error) and never exits the process; mfs_last_error() returns the
+ * message of the calling thread's last failure.  Arrays are plain host
+ * pointers in the reference's layout: per-variable blocks (rho,u,v,w,p),
+ * ghost cell of boundary face k at index nTCell+k, res as 5 blocks of
+ * nTCell, DQ and gradients as 5 blocks of nTCell+nBFace.  Device work is
+ * asynchronous on the context's own HIP stream; mfs_sync waits for it.
+ * A context is used by one host thread at a time.
+ */
+#ifndef MFS_H
+#define MFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFS_OK 0
+#define MFS_ERR_ARG (-1)
+#define MFS_ERR_HIP (-2)
+#define MFS_ERR_IO (-3)
+#define MFS_ERR_MESH (-4)
+#define MFS_ERR_STATE (-5)
+#define MFS_ERR_NOMEM (-6)
+
+/* Precision of a context. */
+#define MFS_FP64 0  /* plain FP64, bitwise equal to the reference's serial arithmetic */
+#define MFS_MIXED 1 /* FP32 operator (flux, time step, LU-SGS), FP64 residual/update/norm */
+
+/*
+ * Solver parameters: the mutable globals of src/Base/para_field_global.cpp
+ * (:37-221) gathered into one per-solve struct.  mfs_params_default fills
+ * the reference defaults; mfs_params_derive recomputes the freestream and
+ * limit values from mach/alpha/beita/re/T with the formulas of
+ * para_field_global.cpp:150-209.
+ */
+typedef struct mfs_params {
+    /* inputs */
+    double mach, alpha, beita; /* alpha, beita in degrees */
+    double re, T, gam, gascon;
+    double cfl_start, cfl_end;
+    int cfl_nstep;
+    double epsa_r;      /* entropy-fix constant alf_l = epsa_r */
+    double lhs_omga;    /* LU-SGS lhs ratio */
+    double ratio_dtmax; /* dt_max/dt_min limit */
+    /* derived by mfs_params_derive */
+    double amu, ainf, uqq;
+    double u_s, v_s, w_s, p_s, rho_s, p_bar, gamm1;
+    double p_stag, e_stag, p_min, p_max, rho_min, rho_max, p_break, e_stag_max;
+    double alf_l, alf_n;
+} mfs_params;
+
+void mfs_params_default(mfs_params* p);
+int mfs_params_derive(mfs_params* p);
+
+const char* mfs_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Host grid: the PolyGrid of src/Base/grid_polyhedra.h after the set-up of
+ * src/main.cpp:166-170.  Built and owned by the library.                */
+typedef struct mfs_grid mfs_grid;
+
+typedef struct mfs_grid_info {
+    int nTCell, nTFace, nBFace, nTNode;
+    int preprocessed; /* 1 after mfs_grid_preprocess */
+    int decoupled;    /* 1 after mfs_grid_decouple */
+} mfs_grid_info;
+
+/* Box + Gaussian-bump Kuhn-tetrahedron mesh (unit box, nx*ny*nz hexes x 6
+ * tets, interior nodes jittered by +-jitter*spacing, z=0 wall displaced by
+ * bump_height*exp(-r^2/(2 bump_sigma^2)); patch 1 WALL, patch 2 FAR_FIELD). */
+int mfs_grid_generate_box_bump(int nx, int ny, int nz, double jitter, double bump_height, double bump_sigma,
+                               uint64_t seed, mfs_grid** out);
+/* LoadGrid with explicit paths (load_grid.cpp:82-314: FlowStar binary + bc.para). */
+int mfs_grid_load(const char* mesh_path, const char* bc_path, mfs_grid** out);
+int mfs_grid_save(const mfs_grid* g, const char* mesh_path, const char* bc_path);
+/* Decoupling: TDTree decomposition and cell/node/face renumbering
+ * (load_grid.cpp:339-395 under -DTDTREE).  Must precede preprocess. */
+int mfs_grid_decouple(mfs_grid* g);
+/* PreprocessGrid (load_grid.cpp:405-540): SpecifyBC, ComputeMetrics,
+ * AllocAndInitFlowfield (freestream), SetGhostVariables. */
+int mfs_grid_preprocess(mfs_grid* g, const mfs_params* p);
+int mfs_grid_get_info(const mfs_grid* g, mfs_grid_info* info);
+/* Copies a named grid array ("x","f2c","xfn","area","vol","xcc","rho",
+ * "bctype", "tdtree_leaves", ...) into dst; bytes must equal its size.
+ * mfs_grid_array_bytes returns that size (0 if unknown). */
+size_t mfs_grid_array_bytes(const mfs_grid* g, const char* name);
+int mfs_grid_get_array(const mfs_grid* g, const char* name, void* dst, size_t bytes);
+void mfs_grid_destroy(mfs_grid* g);
+
+/* ------------------------------------------------------------------ */
+/* Mesh view: borrowed host pointers to a preprocessed PolyGrid's arrays
+ * (names as in grid_polyhedra.h).  The context copies what it needs.     */
+typedef struct mfs_mesh_view {
+    int nTCell, nTFace, nBFace, nTNode;
+    const double *x, *y, *z;          /* nTNode */
+    const int* nNPF;                  /* nTFace */
+    const int* f2n;                   /* sum(nNPF) */
+    const int* f2c;                   /* 2*nTFace, ghost of boundary face k = nTCell+k */
+    const int* bc_type;               /* nBFace: WALL=3 SYMM=4 FAR_FIELD=6 ... */
+    const double *xfn, *yfn, *zfn, *area; /* nTFace */
+    const double *xfc, *yfc, *zfc;    /* nTFace */
+    const double *xcc, *ycc, *zcc, *vol; /* nTCell+nBFace */
+    const double *rho, *u, *v, *w, *p; /* nTCell+nBFace, p is gauge p - p_bar */
+} mfs_mesh_view;
+
+int mfs_grid_view(const mfs_grid* g, mfs_mesh_view* view);
+
+/* ------------------------------------------------------------------ */
+/* Device context: all device buffers and one HIP stream on the current
+ * device.                                                               */
+typedef struct mfs_ctx mfs_ctx;
+
+int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, int precision, mfs_ctx** out);
+int mfs_ctx_destroy(mfs_ctx* ctx);
+
+/* The kernel entry points, one per reference function. */
+int mfs_zero_residuals(mfs_ctx* ctx);     /* cal_flux.cpp:54-72 */
+int mfs_inviscid_flux(mfs_ctx* ctx);      /* cal_flux.cpp:84-192, 1219-1254 (res -= / += F) */
+int mfs_compute_timestep(mfs_ctx* ctx);   /* compute_timestep.cpp:66-122, 756-998, 1315-1338 */
+int mfs_forward_lusgs(mfs_ctx* ctx);      /* forward_lusgs.cpp:67-163 incl. sweeps and update */
+int mfs_pass_interface_data(mfs_ctx* ctx); /* load_grid.cpp:912-943: single domain, no-op */
+int mfs_calculate_gradient(mfs_ctx* ctx); /* cal_gradient.cpp:87-167 */
+/* iter_done++ of main.cpp:289 and one history record (norms of the last
+ * residual, running dt_max/dt_min). */
+int mfs_iteration_done(mfs_ctx* ctx);
+
+/* n iterations of the driver loop (main.cpp:201-291), fused kernels;
+ * appends n history records. */
+int mfs_step(mfs_ctx* ctx, int n);
+int mfs_sync(mfs_ctx* ctx);
+
+/* Per-iteration history: record k = {norm_rho, norm_mx, norm_my, norm_mz,
+ * norm_et, dt_max, dt_min} (main.cpp:294-329 evaluated every iteration).
+ * Copies min(max_records, available) records; returns the count (>=0). */
+int mfs_history_count(mfs_ctx* ctx);
+int mfs_get_history(mfs_ctx* ctx, double* out, int max_records);
+
+/* Field ids for download/upload. */
+#define MFS_F_RHO 0
+#define MFS_F_U 1
+#define MFS_F_V 2
+#define MFS_F_W 3
+#define MFS_F_P 4
+#define MFS_F_RES 5  /* 5*nTCell */
+#define MFS_F_DT 6   /* nTCell */
+#define MFS_F_DQ 7   /* 5*(nTCell+nBFace) */
+#define MFS_F_DQDX 8 /* 5*(nTCell+nBFace) */
+#define MFS_F_DQDY 9
+#define MFS_F_DQDZ 10
+#define MFS_F_DIAG 11 /* nTCell, LU-SGS diagonal of the last ForwardLUSGS */
+#define MFS_F_QNODE 12 /* 5*nTNode node values of the last gradient */
+size_t mfs_field_count(mfs_ctx* ctx, int field);
+int mfs_download(mfs_ctx* ctx, int field, double* dst, size_t count);
+int mfs_upload(mfs_ctx* ctx, int field, const double* src, size_t count); /* state fields only */
+
+/* Scalars of the running solve. */
+typedef struct mfs_scalars {
+    int iter_done;
+    double dt_max, dt_min, cfl_min;
+} mfs_scalars;
+int mfs_get_scalars(mfs_ctx* ctx, mfs_scalars* s);
+
+/* Timing and schedule facts for the bench and tests. */
+typedef struct mfs_ctx_info {
+    int nTCell, nTFace, nBFace, nTNode;
+    int precision;
+    int sweep_levels;      /* DAG depth of the LU-SGS sweep order */
+    size_t device_bytes;   /* device memory held by the context */
+} mfs_ctx_info;
+int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
+/* Kernel time accounting: when enabled, every launch of the named kernel
+ * group is bracketed with HIP events on the context stream. */
+int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
+/* names: "flux","timestep","reduce","fwd_sweep","bwd_sweep","node","gradient","total";
+ * returns accumulated milliseconds and launch count. */
+int mfs_ctx_get_timing(mfs_ctx* ctx, const char* name, double* ms, int* launches);
+int mfs_ctx_reset_timing(mfs_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFS_H */

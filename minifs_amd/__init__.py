@@ -1,0 +1,210 @@
+"""MI355X-native MiniFS solver path.
+
+Host-side mirror of the reference's set-up and kernel interface
+(src/Kernels/load_grid.h, cal_flux.h, compute_timestep.h, forward_lusgs.h,
+cal_gradient.h; driver loop src/main.cpp:198-291) over the C ABI of
+libmfs_hip.so.  Grid is the PolyGrid after LoadGrid/Decoupling/
+PreprocessGrid; Solver is the device context on which the kernel entry
+points run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD_IDS, MFS_FP64, MFS_MIXED, MfsError, check, lib
+
+__all__ = ["Grid", "Solver", "make_params", "MFS_FP64", "MFS_MIXED", "MfsError", "lib"]
+
+BC_WALL, BC_SYMM, BC_FAR_FIELD, BC_FAR_FIELD_MOD, BC_INTERFACE = 3, 4, 6, 61, 10
+
+
+def make_params(mach: float = 0.8395, alpha: float = 3.06, cfl: float | None = None, **over) -> _lib.MfsParams:
+    """Reference defaults (para_field_global.cpp:37-146) with the flow
+    condition overridden, then the derived values recomputed
+    (para_field_global.cpp:150-209)."""
+    p = _lib.MfsParams()
+    lib().mfs_params_default(C.byref(p))
+    p.mach = mach
+    p.alpha = alpha
+    if cfl is not None:
+        p.cfl_start = cfl
+        p.cfl_end = cfl
+    for k, v in over.items():
+        setattr(p, k, v)
+    check(lib().mfs_params_derive(C.byref(p)), "mfs_params_derive")
+    return p
+
+
+_INT_ARRAYS = {"nNPF", "f2n", "f2c", "bctype", "cellPerm", "facePerm", "nodePerm", "tdtree_nodes"}
+
+
+class Grid:
+    """Host grid: FlowStar mesh + boundary conditions, optionally TDTree
+    renumbered (Decoupling) and preprocessed (metrics, freestream, ghosts)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def box_bump(cls, nx: int, ny: int | None = None, nz: int | None = None, jitter: float = 0.1,
+                 bump_height: float = 0.05, bump_sigma: float = 0.1, seed: int = 20260101) -> "Grid":
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        h = C.c_void_p()
+        check(lib().mfs_grid_generate_box_bump(nx, ny, nz, jitter, bump_height, bump_sigma, seed, C.byref(h)),
+              "mfs_grid_generate_box_bump")
+        return cls(h)
+
+    @classmethod
+    def load(cls, mesh_path: str, bc_path: str) -> "Grid":
+        h = C.c_void_p()
+        check(lib().mfs_grid_load(mesh_path.encode(), bc_path.encode(), C.byref(h)), "mfs_grid_load")
+        return cls(h)
+
+    def save(self, mesh_path: str, bc_path: str) -> None:
+        check(lib().mfs_grid_save(self._h, mesh_path.encode(), bc_path.encode()), "mfs_grid_save")
+
+    def decouple(self) -> "Grid":
+        check(lib().mfs_grid_decouple(self._h), "mfs_grid_decouple")
+        return self
+
+    def preprocess(self, params: _lib.MfsParams) -> "Grid":
+        check(lib().mfs_grid_preprocess(self._h, C.byref(params)), "mfs_grid_preprocess")
+        return self
+
+    @property
+    def info(self) -> dict:
+        i = _lib.MfsGridInfo()
+        check(lib().mfs_grid_get_info(self._h, C.byref(i)), "mfs_grid_get_info")
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def array(self, name: str) -> np.ndarray:
+        nbytes = lib().mfs_grid_array_bytes(self._h, name.encode())
+        dt = np.int32 if name in _INT_ARRAYS else np.float64
+        out = np.empty(nbytes // np.dtype(dt).itemsize, dtype=dt)
+        if nbytes:
+            check(lib().mfs_grid_get_array(self._h, name.encode(), out.ctypes.data_as(C.c_void_p), nbytes),
+                  "mfs_grid_get_array")
+        return out
+
+    def view(self) -> _lib.MfsMeshView:
+        v = _lib.MfsMeshView()
+        check(lib().mfs_grid_view(self._h, C.byref(v)), "mfs_grid_view")
+        return v
+
+    def close(self) -> None:
+        if self._h:
+            lib().mfs_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """Device context with the reference's kernel entry points."""
+
+    def __init__(self, grid: Grid, params: _lib.MfsParams, precision: int = MFS_FP64):
+        self._h = C.c_void_p()
+        view = grid.view()
+        self.params = params
+        check(lib().mfs_ctx_create(C.byref(view), C.byref(params), precision, C.byref(self._h)), "mfs_ctx_create")
+        self.grid = grid
+
+    # -- the reference's per-iteration kernels (main.cpp:201-291)
+    def zero_residuals(self):
+        check(lib().mfs_zero_residuals(self._h), "mfs_zero_residuals")
+
+    def inviscid_flux(self):
+        check(lib().mfs_inviscid_flux(self._h), "mfs_inviscid_flux")
+
+    def compute_timestep(self):
+        check(lib().mfs_compute_timestep(self._h), "mfs_compute_timestep")
+
+    def forward_lusgs(self):
+        check(lib().mfs_forward_lusgs(self._h), "mfs_forward_lusgs")
+
+    def pass_interface_data(self):
+        check(lib().mfs_pass_interface_data(self._h), "mfs_pass_interface_data")
+
+    def calculate_gradient(self):
+        check(lib().mfs_calculate_gradient(self._h), "mfs_calculate_gradient")
+
+    def iteration_done(self):
+        check(lib().mfs_iteration_done(self._h), "mfs_iteration_done")
+
+    def iterate_reference_order(self, n: int = 1):
+        """One driver-loop iteration through the individual entry points."""
+        for _ in range(n):
+            self.zero_residuals()
+            self.inviscid_flux()
+            self.compute_timestep()
+            self.forward_lusgs()
+            self.pass_interface_data()
+            self.calculate_gradient()
+            self.iteration_done()
+
+    def step(self, n: int = 1):
+        check(lib().mfs_step(self._h, n), "mfs_step")
+
+    def sync(self):
+        check(lib().mfs_sync(self._h), "mfs_sync")
+
+    def history(self) -> np.ndarray:
+        n = lib().mfs_history_count(self._h)
+        out = np.zeros((n, 7), dtype=np.float64)
+        if n:
+            got = check(lib().mfs_get_history(self._h, out.ctypes.data_as(_lib._DP), n), "mfs_get_history")
+            out = out[:got]
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        fid = FIELD_IDS[name]
+        n = lib().mfs_field_count(self._h, fid)
+        out = np.empty(n, dtype=np.float64)
+        check(lib().mfs_download(self._h, fid, out.ctypes.data_as(_lib._DP), n), "mfs_download")
+        return out
+
+    def upload(self, name: str, values: np.ndarray):
+        fid = FIELD_IDS[name]
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().mfs_upload(self._h, fid, a.ctypes.data_as(_lib._DP), a.size), "mfs_upload")
+
+    def scalars(self) -> dict:
+        s = _lib.MfsScalars()
+        check(lib().mfs_get_scalars(self._h, C.byref(s)), "mfs_get_scalars")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def info(self) -> dict:
+        i = _lib.MfsCtxInfo()
+        check(lib().mfs_ctx_get_info(self._h, C.byref(i)), "mfs_ctx_get_info")
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def enable_timing(self, on: bool = True):
+        check(lib().mfs_ctx_enable_timing(self._h, 1 if on else 0), "mfs_ctx_enable_timing")
+
+    def timing(self, name: str) -> tuple[float, int]:
+        ms = C.c_double()
+        cnt = C.c_int()
+        check(lib().mfs_ctx_get_timing(self._h, name.encode(), C.byref(ms), C.byref(cnt)), "mfs_ctx_get_timing")
+        return ms.value, cnt.value
+
+    def reset_timing(self):
+        check(lib().mfs_ctx_reset_timing(self._h), "mfs_ctx_reset_timing")
+
+    def close(self):
+        if self._h:
+            lib().mfs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
