@@ -1,0 +1,58 @@
+/*
+ * Test infrastructure (oracle): a plain-C CPU restatement of the MiniFS
+ * solver path, used only by tests/, __graft_entry__.smoke() and the CPU
+ * leg of bench.py, and only as the checker or the CPU baseline.
+ *
+ * It restates, from the reference sources (paths relative to the MiniFS
+ * tree): the FlowStar reader (src/Kernels/load_grid.cpp:129-314), SpecifyBC
+ * (:857-902), ComputeMetrics (src/Base/grid_polyhedra.cpp:1071-1100 and
+ * grid_polyhedra_auxiliary_functions.cpp:68-808), AllocAndInitFlowfield and
+ * SetGhostVariables (load_grid.cpp:571-601, 953-1212), ComputeWeight3D_Node
+ * (grid_polyhedra_auxiliary_functions.cpp:831-1049) and the driver loop of
+ * src/main.cpp:198-291 with ZeroResiduals / InviscidFlux (cal_flux.cpp),
+ * ComputeTimeStep (compute_timestep.cpp), ForwardLUSGS (forward_lusgs.cpp)
+ * and CalculateGradient (cal_gradient.cpp).
+ *
+ * Mode 0 runs the reference's serial loops (face loops scattering into
+ * cells, sequential sweeps).  Mode > 0 runs the same arithmetic with that
+ * many OpenMP threads: per-cell gathers in ascending face order and
+ * level-scheduled sweeps, which add the same terms in the same order.
+ */
+#ifndef MFS_ORACLE_H
+#define MFS_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_grid or_grid;
+
+int or_load(const char* mesh_path, const char* bc_path, or_grid** out);
+/* sets the flow condition (para_field_global.cpp:150-209) and runs the set-up */
+int or_preprocess(or_grid* g, double mach, double alpha, double cfl);
+/* runs iters iterations; hist receives iters*7 doubles
+ * (5 residual norms, running dt_max, dt_min); t_loop the loop seconds */
+int or_run(or_grid* g, int iters, int threads, double* hist, double* t_loop);
+/* copies a named array: rho u v w p (nC+nB), res (5 nC), dt (nC), dq dqdx
+ * dqdy dqdz (5 (nC+nB)), xfn..zfn area xfc..zfc (nF), xcc..zcc vol (nC+nB) */
+int or_get(const or_grid* g, const char* name, double* dst, size_t count);
+int or_sizes(const or_grid* g, int* nC, int* nF, int* nB, int* nN);
+void or_free(or_grid* g);
+const char* or_error(void);
+
+/* Point functions for property tests.  derived[] receives u_s v_s w_s
+ * rho_s p_bar p_stag e_stag p_min p_max rho_min rho_max p_break e_stag_max
+ * gamm1 for the given flow condition (para_field_global.cpp:150-209). */
+void or_derived(double mach, double alpha, double derived[14]);
+/* RoeFlux_noprec for one face (cal_flux.cpp:682-833) */
+void or_roe_flux(const double ql[5], const double qr[5], double nx, double ny, double nz, double area, double mach,
+                 double alpha, double f[5]);
+/* FluxLUSGS3D (forward_lusgs.cpp:1271-1324) */
+void or_flux_lusgs(const double q[5], const double dq[5], double nx, double ny, double nz, double mach, double alpha,
+                   double f[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

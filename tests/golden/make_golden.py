@@ -1,0 +1,86 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's own
+serial build (oracle/_ref/ref_driver, compiled from the reference sources by
+oracle/Makefile) on meshes made by the box+bump generator.
+
+Fixtures (all little-endian raw arrays, plus manifest.json):
+  box4_*      : 4x4x4x6 = 384-cell mesh (FlowStar file + bc.para) and the
+                reference's geometry, initial state and state after 1 and 10
+                iterations at Mach 0.8, AoA 1.25, CFL 10, with its history.
+  box20_*     : config-1 mesh (20x20x20x6 = 48,000 cells, seed 20260101),
+                Mach 0.5, AoA 0, CFL 10: the reference's 100-iteration
+                history and SHA-256 digests of its state after 1 and 100
+                iterations (the arrays themselves are too large to commit).
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+import numpy as np  # noqa: E402
+
+import minifs_amd as mfs  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+FIELDS = ["rho", "u", "v", "w", "p", "res", "dt", "dq", "dqdx", "dqdy", "dqdz"]
+GEO = ["xfn", "yfn", "zfn", "area", "xfc", "yfc", "zfc", "xcc", "ycc", "zcc", "vol"]
+
+
+def sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def run_ref(mesh, bc, out, mach, alpha, cfl, iters, dumps):
+    os.makedirs(out, exist_ok=True)
+    subprocess.run([REF, mesh, bc, out, repr(mach), repr(alpha), repr(cfl), str(iters), ",".join(map(str, dumps))],
+                   check=True, stdout=subprocess.DEVNULL)
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("build oracle/_ref/ref_driver first (make -C oracle ref)")
+    man = {"generator": "minifs_amd.Grid.box_bump(n, seed=20260101).decouple()", "cases": {}}
+    with tempfile.TemporaryDirectory() as td:
+        # ---- box4: full arrays
+        g = mfs.Grid.box_bump(4).decouple()
+        mesh, bc = os.path.join(HERE, "box4.mfl"), os.path.join(HERE, "box4.bc")
+        g.save(mesh, bc)
+        out = os.path.join(td, "box4")
+        run_ref(mesh, bc, out, 0.8, 1.25, 10.0, 10, (0, 1, 10))
+        for nm in GEO:
+            shutil.copy(os.path.join(out, f"geo_{nm}.f64"), os.path.join(HERE, f"box4_geo_{nm}.f64"))
+        for it in (0, 1, 10):
+            for nm in FIELDS:
+                src = os.path.join(out, f"it{it}_{nm}.f64")
+                if os.path.exists(src):
+                    shutil.copy(src, os.path.join(HERE, f"box4_it{it}_{nm}.f64"))
+        shutil.copy(os.path.join(out, "history.f64"), os.path.join(HERE, "box4_history.f64"))
+        man["cases"]["box4"] = {"n": 4, "mach": 0.8, "alpha": 1.25, "cfl": 10.0, "iters": 10,
+                                "mesh_sha256": sha(mesh), "dumps": [0, 1, 10]}
+        # ---- box20 (config 1): history + digests
+        g = mfs.Grid.box_bump(20).decouple()
+        mesh, bc = os.path.join(td, "box20.mfl"), os.path.join(td, "box20.bc")
+        g.save(mesh, bc)
+        out = os.path.join(td, "box20")
+        run_ref(mesh, bc, out, 0.5, 0.0, 10.0, 100, (1, 100))
+        shutil.copy(os.path.join(out, "history.f64"), os.path.join(HERE, "box20_history.f64"))
+        dig = {f"it{it}_{nm}": sha(os.path.join(out, f"it{it}_{nm}.f64")) for it in (1, 100) for nm in FIELDS}
+        man["cases"]["box20"] = {"n": 20, "mach": 0.5, "alpha": 0.0, "cfl": 10.0, "iters": 100,
+                                 "mesh_sha256": sha(mesh), "state_sha256": dig,
+                                 "final_line": open(os.path.join(out, "summary.txt")).read().splitlines()[-1]}
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump(man, f, indent=1, sort_keys=True)
+    print("golden fixtures written")
+
+
+if __name__ == "__main__":
+    main()

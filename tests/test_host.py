@@ -1,0 +1,219 @@
+"""Host side of the product library (no GPU): the C ABI exports, the mesh
+generator and FlowStar I/O, the TDTree renumbering, and the set-up
+(PreprocessGrid) against the reference's golden geometry."""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import minifs_amd as mfs
+from minifs_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+MAN = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "mfs.h")).read()
+    declared = set(re.findall(r"\b(mfs_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) >= 30
+    L = C.CDLL(_lib.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python binding covers the same set
+    assert declared == set(_lib._SIGS), declared ^ set(_lib._SIGS)
+
+
+def test_params_known_answers_and_errors():
+    """mfs_params_derive restates para_field_global.cpp:150-209; the m6wing
+    values in its comments (rho_s corrected, see test_oracle)."""
+    p = mfs.make_params(mach=0.8395, alpha=3.06)
+    for k, v in {"u_s": 285.27, "w_s": 15.2499, "rho_s": 1.13625, "p_bar": 93984.5, "e_stag": 281327,
+                 "p_min": -93975.1, "p_max": 1.39707e6, "rho_min": 0.000113625, "rho_max": 15.7995,
+                 "p_break": -93044.7, "e_stag_max": 2.81327e10}.items():
+        assert abs(getattr(p, k) - v) <= 6e-6 * abs(v), k
+    with pytest.raises(mfs.MfsError):
+        mfs.make_params(mach=-1.0)
+
+
+@pytest.mark.parametrize("n", [(1, 1, 1), (3, 2, 4), (6, 6, 6)])
+def test_generator_counts_and_orientation(n):
+    nx, ny, nz = n
+    g = mfs.Grid.box_bump(nx, ny, nz)
+    info = g.info
+    assert info["nTCell"] == 6 * nx * ny * nz
+    assert info["nTNode"] == (nx + 1) * (ny + 1) * (nz + 1)
+    assert info["nBFace"] == 4 * (nx * ny + ny * nz + nx * nz)
+    assert info["nTFace"] == (4 * info["nTCell"] + info["nBFace"]) // 2
+    f2c = g.array("f2c").reshape(-1, 2)
+    nB = info["nBFace"]
+    assert (f2c[:nB, 1] < 0).all() and (f2c[nB:, 1] >= 0).all()
+    # patch 1 = the z=0 wall: 2 nx ny faces
+    assert (f2c[:nB, 1] == -1).sum() == 2 * nx * ny
+    # right-hand normals from c1 to c2 make every volume positive
+    g.preprocess(mfs.make_params(0.5, 0.0, 10.0))
+    assert (g.array("vol") > 0).all()
+
+
+def test_generator_is_deterministic_and_seeded():
+    a = mfs.Grid.box_bump(5, seed=3).array("x")
+    b = mfs.Grid.box_bump(5, seed=3).array("x")
+    c = mfs.Grid.box_bump(5, seed=4).array("x")
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
+def test_flowstar_round_trip(tmp_path):
+    g = mfs.Grid.box_bump(4, 3, 2).decouple()
+    mesh, bc = str(tmp_path / "a.mfl"), str(tmp_path / "a.bc")
+    g.save(mesh, bc)
+    h = mfs.Grid.load(mesh, bc)
+    for nm in ("x", "y", "z", "nNPF", "f2n", "f2c"):
+        np.testing.assert_array_equal(g.array(nm), h.array(nm), err_msg=nm)
+    assert g.info["nBFace"] == h.info["nBFace"]
+
+
+def test_loader_rejects_bad_files(tmp_path):
+    with pytest.raises(mfs.MfsError):
+        mfs.Grid.load(str(tmp_path / "missing.mfl"), str(tmp_path / "missing.bc"))
+    p = tmp_path / "trunc.mfl"
+    p.write_bytes(np.array([10, 10, 10], np.int32).tobytes() + b"\0" * 16)
+    (tmp_path / "x.bc").write_text("1\n1 3\n")
+    with pytest.raises(mfs.MfsError):
+        mfs.Grid.load(str(p), str(tmp_path / "x.bc"))
+
+
+def test_loader_reorients_first_side_boundary_faces(tmp_path):
+    """load_grid.cpp:214-224: a face stored with the boundary as its first
+    side has nodes 1..n-1 reversed and its sides swapped on reading."""
+    g = mfs.Grid.box_bump(2)
+    mesh, bc = str(tmp_path / "a.mfl"), str(tmp_path / "a.bc")
+    g.save(mesh, bc)
+    raw = bytearray(open(mesh, "rb").read())
+    info = g.info
+    nN, nF = info["nTNode"], info["nTFace"]
+    off_f2n = 12 + 24 * nN + 4 * nF
+    off_f2c = off_f2n + 4 * 3 * nF
+    f2n = np.frombuffer(raw, np.int32, 3 * nF, off_f2n).copy().reshape(-1, 3)
+    f2c = np.frombuffer(raw, np.int32, 2 * nF, off_f2c).copy().reshape(-1, 2)
+    f2c[0] = f2c[0, ::-1]  # boundary side first
+    f2n[0, 1:] = f2n[0, 1:][::-1]
+    raw[off_f2n:off_f2n + f2n.nbytes] = f2n.tobytes()
+    raw[off_f2c:off_f2c + f2c.nbytes] = f2c.tobytes()
+    p2 = tmp_path / "b.mfl"
+    p2.write_bytes(bytes(raw))
+    h = mfs.Grid.load(str(p2), bc)
+    np.testing.assert_array_equal(h.array("f2n"), g.array("f2n"))
+    np.testing.assert_array_equal(h.array("f2c"), g.array("f2c"))
+
+
+def _tree(g):
+    t = g.array("tdtree_nodes").reshape(-1, 14)
+    return [dict(zip("fc lc ff lf fn ln leaf iso nparts isonode s0 s1 s2 s3".split(), map(int, r))) for r in t]
+
+
+def test_tdtree_renumbering_invariants():
+    """Restated TDTree_creation (TDTree_creation.cpp:131-519): permutations
+    are bijections, boundary faces stay first, parts precede their
+    separator, cells of different parts never share a face, and a cell keeps
+    the relative order of its part."""
+    g0 = mfs.Grid.box_bump(12)
+    f2c0 = g0.array("f2c").reshape(-1, 2)
+    g = mfs.Grid.box_bump(12).decouple()
+    nC, nF, nB, nN = (g.info[k] for k in ("nTCell", "nTFace", "nBFace", "nTNode"))
+    cp, fp, npm = g.array("cellPerm"), g.array("facePerm"), g.array("nodePerm")
+    for perm, n in ((cp, nC), (fp, nF), (npm, nN)):
+        assert np.array_equal(np.sort(perm), np.arange(n))
+    assert (fp[:nB] < nB).all() and (fp[nB:] >= nB).all()
+    f2c = g.array("f2c").reshape(-1, 2)
+    # faces keep their cells through the renumbering
+    np.testing.assert_array_equal(f2c[fp[nB:], 0], cp[f2c0[nB:, 0]])
+    nodes = _tree(g)
+    cell_tree = nodes[nodes[0]["isonode"]]["isonode"]
+
+    def walk(i, leaves):
+        d = nodes[i]
+        if d["leaf"]:
+            leaves.append(i)
+            return
+        sons = [d[f"s{k}"] for k in range(4) if d[f"s{k}"] >= 0]
+        prev = d["fc"]
+        for s in sons:
+            assert nodes[s]["fc"] == prev
+            prev = nodes[s]["lc"] + 1
+            walk(s, leaves)
+        if d["isonode"] >= 0 and d["nparts"] > 0:
+            assert nodes[d["isonode"]]["fc"] == prev and nodes[d["isonode"]]["lc"] == d["lc"]
+            walk(d["isonode"], leaves)
+        # cells of different sons are not face neighbours
+        owner = {}
+        for k, s in enumerate(sons):
+            for c in range(nodes[s]["fc"], nodes[s]["lc"] + 1):
+                owner[c] = k
+        inner = f2c[nB:]
+        a = np.array([owner.get(int(x), -1) for x in inner[:, 0]])
+        b = np.array([owner.get(int(x), -1) for x in inner[:, 1]])
+        both = (a >= 0) & (b >= 0)
+        assert (a[both] == b[both]).all()
+
+    leaves = []
+    walk(cell_tree, leaves)
+    covered = np.zeros(nC, int)
+    for lf in leaves:
+        covered[nodes[lf]["fc"]:nodes[lf]["lc"] + 1] += 1
+        rng = np.arange(nodes[lf]["fc"], nodes[lf]["lc"] + 1)
+        if rng.size > 1:  # stable: a leaf keeps the original relative order
+            orig = np.argsort(cp)[rng]
+            assert (np.diff(orig) > 0).all()
+    assert (covered == 1).all()
+
+
+def test_preprocess_matches_reference_geometry_bitwise(tmp_path):
+    """PreprocessGrid restated in C++ reproduces the reference's metrics,
+    freestream and ghost state bit for bit (golden box4 from the reference
+    build)."""
+    c = MAN["cases"]["box4"]
+    g = mfs.Grid.load(os.path.join(GOLD, "box4.mfl"), os.path.join(GOLD, "box4.bc"))
+    g.preprocess(mfs.make_params(c["mach"], c["alpha"], c["cfl"]))
+    for nm in ("xfn", "yfn", "zfn", "area", "xfc", "yfc", "zfc", "xcc", "ycc", "zcc", "vol"):
+        np.testing.assert_array_equal(g.array(nm), np.fromfile(os.path.join(GOLD, f"box4_geo_{nm}.f64")), err_msg=nm)
+    for nm in ("rho", "u", "v", "w", "p"):
+        np.testing.assert_array_equal(g.array(nm), np.fromfile(os.path.join(GOLD, f"box4_it0_{nm}.f64")), err_msg=nm)
+
+
+def test_preprocess_matches_oracle_on_config1(tmp_path):
+    from oracle import Oracle
+
+    g = mfs.Grid.box_bump(20).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    g.preprocess(mfs.make_params(0.5, 0.0, 10.0))
+    o = Oracle(mesh, bc, 0.5, 0.0, 10.0)
+    for nm in ("xfn", "area", "xcc", "zcc", "vol", "rho", "u", "w", "p"):
+        np.testing.assert_array_equal(g.array(nm), o.get(nm), err_msg=nm)
+
+
+def test_solver_without_gpu_fails_loudly():
+    """No silent CPU fallback: creating a device context needs a GPU."""
+    from tests.conftest import gpu_available
+
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    g = mfs.Grid.box_bump(2).decouple().preprocess(mfs.make_params(0.5, 0.0, 10.0))
+    with pytest.raises(mfs.MfsError):
+        mfs.Solver(g, mfs.make_params(0.5, 0.0, 10.0))
+
+
+def test_preprocess_rejects_unknown_patch(tmp_path):
+    g = mfs.Grid.box_bump(2)
+    mesh, bc = str(tmp_path / "a.mfl"), str(tmp_path / "a.bc")
+    g.save(mesh, bc)
+    open(bc, "w").write("1\n1 3\n")  # patch 2 missing
+    h = mfs.Grid.load(mesh, bc)
+    with pytest.raises(mfs.MfsError):
+        h.preprocess(mfs.make_params(0.5, 0.0, 10.0))

@@ -1,0 +1,258 @@
+"""The oracle (C restatement, oracle/mfs_oracle.c) pinned against the
+reference's own outputs (golden fixtures from the reference's serial build)
+and against closed forms the mathematics fixes."""
+import ctypes as C
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import Oracle, lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+MAN = json.load(open(os.path.join(GOLD, "manifest.json")))
+FIELDS = ["rho", "u", "v", "w", "p", "res", "dt", "dq", "dqdx", "dqdy", "dqdz"]
+GEO = ["xfn", "yfn", "zfn", "area", "xfc", "yfc", "zfc", "xcc", "ycc", "zcc", "vol"]
+
+
+def gold(name):
+    return np.fromfile(os.path.join(GOLD, name))
+
+
+def box4_oracle():
+    c = MAN["cases"]["box4"]
+    return Oracle(os.path.join(GOLD, "box4.mfl"), os.path.join(GOLD, "box4.bc"), c["mach"], c["alpha"], c["cfl"])
+
+
+def test_box4_geometry_and_initial_state_bitwise():
+    o = box4_oracle()
+    for nm in GEO:
+        np.testing.assert_array_equal(o.get(nm), gold(f"box4_geo_{nm}.f64"), err_msg=nm)
+    for nm in ["rho", "u", "v", "w", "p"]:
+        np.testing.assert_array_equal(o.get(nm), gold(f"box4_it0_{nm}.f64"), err_msg=nm)
+
+
+@pytest.mark.parametrize("threads", [0, 4])
+def test_box4_iterations_bitwise(threads):
+    o = box4_oracle()
+    h1, _ = o.run(1, threads)
+    for nm in FIELDS:
+        np.testing.assert_array_equal(o.get(nm), gold(f"box4_it1_{nm}.f64"), err_msg=f"it1 {nm}")
+    h9, _ = o.run(9, threads)
+    for nm in FIELDS:
+        np.testing.assert_array_equal(o.get(nm), gold(f"box4_it10_{nm}.f64"), err_msg=f"it10 {nm}")
+    np.testing.assert_array_equal(np.vstack([h1, h9]), gold("box4_history.f64").reshape(-1, 7))
+
+
+def test_config1_history_and_digests(tmp_path):
+    """Config 1 (48,000 cells, 100 iterations): the generator reproduces the
+    reference run's mesh file, and the oracle reproduces the reference's
+    history and final state bit for bit."""
+    import minifs_amd as mfs
+
+    c = MAN["cases"]["box20"]
+    g = mfs.Grid.box_bump(20).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    assert hashlib.sha256(open(mesh, "rb").read()).hexdigest() == c["mesh_sha256"]
+    o = Oracle(mesh, bc, c["mach"], c["alpha"], c["cfl"])
+    h, _ = o.run(100, min(8, os.cpu_count() or 1))
+    np.testing.assert_array_equal(h, gold("box20_history.f64").reshape(-1, 7))
+    for nm in FIELDS:
+        d = hashlib.sha256(o.get(nm).tobytes()).hexdigest()
+        assert d == c["state_sha256"][f"it100_{nm}"], nm
+    # the reference prints its final line with 6 significant digits (main.cpp:319-329)
+    last = c["final_line"].split()
+    assert int(last[0]) == 100
+    np.testing.assert_allclose(h[-1], [float(x) for x in last[1:]], rtol=5e-6)
+
+
+def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver):
+    """Where the reference build exists: a mesh not in the fixtures, other flow."""
+    import minifs_amd as mfs
+    from tests.refrun import load_state, run_reference
+
+    g = mfs.Grid.box_bump(5, 3, 4, seed=7).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    ref = run_reference(ref_driver, mesh, bc, str(tmp_path / "r"), 0.7, -2.0, 5.0, 6, dumps=(6,))
+    o = Oracle(mesh, bc, 0.7, -2.0, 5.0)
+    h, _ = o.run(6, 2)
+    np.testing.assert_array_equal(h, ref["history"])
+    for nm in FIELDS:
+        np.testing.assert_array_equal(o.get(nm), load_state(str(tmp_path / "r"), 6, nm), err_msg=nm)
+
+
+# ---------------------------------------------------------------- closed forms
+def derived(mach, alpha):
+    d = np.zeros(14)
+    L = lib()
+    L.or_derived.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+    L.or_derived.restype = None
+    L.or_derived(mach, alpha, d.ctypes.data_as(C.POINTER(C.c_double)))
+    return dict(zip("u_s v_s w_s rho_s p_bar p_stag e_stag p_min p_max rho_min rho_max p_break e_stag_max gamm1".split(),
+                    d))
+
+
+def test_derived_parameters_known_answers():
+    """The m6wing values written beside the formulas in
+    src/Base/para_field_global.cpp:165-184 (6 significant digits).  The
+    table's rho_s reads 1.12625, but its own p_bar = rho_s*gascon*T = 93984.5
+    and rho_min = 1e-4*rho_s = 0.000113625 both fix rho_s = 1.13625: the
+    comment has a digit slip, and 1.13625 is checked."""
+    d = derived(0.8395, 3.06)
+    kat = {"u_s": 285.27, "w_s": 15.2499, "rho_s": 1.13625, "p_bar": 93984.5, "p_stag": 55120.5, "e_stag": 281327,
+           "p_min": -93975.1, "p_max": 1.39707e6, "rho_min": 0.000113625, "rho_max": 15.7995, "p_break": -93044.7,
+           "e_stag_max": 2.81327e10}
+    for k, v in kat.items():
+        assert abs(d[k] - v) <= 6e-6 * abs(v), (k, d[k], v)
+    assert d["v_s"] == 0.0
+
+
+def _roe(ql, qr, n, area, mach=0.5, alpha=0.0):
+    L = lib()
+    P = C.POINTER(C.c_double)
+    L.or_roe_flux.argtypes = [P, P] + [C.c_double] * 6 + [P]
+    L.or_roe_flux.restype = None
+    f = np.zeros(5)
+    ql, qr = np.ascontiguousarray(ql, float), np.ascontiguousarray(qr, float)
+    L.or_roe_flux(ql.ctypes.data_as(P), qr.ctypes.data_as(P), *n, area, mach, alpha, f.ctypes.data_as(P))
+    return f
+
+
+def _flux_lusgs(q, dq, n, mach=0.5, alpha=0.0):
+    L = lib()
+    P = C.POINTER(C.c_double)
+    L.or_flux_lusgs.argtypes = [P, P] + [C.c_double] * 5 + [P]
+    L.or_flux_lusgs.restype = None
+    f = np.zeros(5)
+    q, dq = np.ascontiguousarray(q, float), np.ascontiguousarray(dq, float)
+    L.or_flux_lusgs(q.ctypes.data_as(P), dq.ctypes.data_as(P), *n, mach, alpha, f.ctypes.data_as(P))
+    return f
+
+
+def phys_flux(q, n, p_bar, gam=1.4):
+    """Normal Euler flux of primitive state q = (rho,u,v,w,gauge p).  The
+    momentum flux carries the gauge pressure (the constant p_bar integrates
+    to zero over a closed cell), the energy flux the absolute enthalpy."""
+    rho, u, v, w, pg = q
+    p = pg + p_bar
+    vn = u * n[0] + v * n[1] + w * n[2]
+    E = p / (gam - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.array([rho * vn, rho * u * vn + pg * n[0], rho * v * vn + pg * n[1], rho * w * vn + pg * n[2],
+                     (E + p) * vn])
+
+
+def random_state(rng, p_bar):
+    return np.array([rng.uniform(0.5, 3.0), *rng.uniform(-150, 150, 3), rng.uniform(-0.3, 0.3) * p_bar])
+
+
+def unit(v):
+    return v / np.linalg.norm(v)
+
+
+def test_roe_consistency_and_antisymmetry():
+    rng = np.random.default_rng(1)
+    p_bar = derived(0.5, 0.0)["p_bar"]
+    for _ in range(50):
+        q = random_state(rng, p_bar)
+        n = unit(rng.normal(size=3))
+        area = rng.uniform(0.1, 2.0)
+        # F(q,q) = A * F_phys(q): the dissipation vanishes for equal states
+        np.testing.assert_allclose(_roe(q, q, n, area), area * phys_flux(q, n, p_bar), rtol=1e-11, atol=1e-9)
+        q2 = random_state(rng, p_bar)
+        # conservation across a face: F(ql,qr,n) = -F(qr,ql,-n)
+        np.testing.assert_allclose(_roe(q, q2, n, area), -_roe(q2, q, -n, area), rtol=1e-10, atol=1e-8)
+
+
+def test_roe_frame_invariance():
+    """The entropy fix scales with |u|+|v|+|w|+c (cal_flux.cpp:779), an L1
+    norm, so the flux is invariant under the axis permutations and
+    reflections that preserve it (not under general rotations)."""
+    rng = np.random.default_rng(2)
+    p_bar = derived(0.5, 0.0)["p_bar"]
+    for _ in range(20):
+        ql, qr = random_state(rng, p_bar), random_state(rng, p_bar)
+        n = unit(rng.normal(size=3))
+        A = np.eye(3)[rng.permutation(3)] * rng.choice([-1.0, 1.0], size=3)[:, None]
+        rot = lambda q: np.array([q[0], *(A @ q[1:4]), q[4]])  # noqa: E731
+        f = _roe(ql, qr, n, 1.0)
+        g = _roe(rot(ql), rot(qr), A @ n, 1.0)
+        np.testing.assert_allclose(g[[0, 4]], f[[0, 4]], rtol=1e-10, atol=1e-8)
+        np.testing.assert_allclose(g[1:4], A @ f[1:4], rtol=1e-9, atol=1e-6)
+
+
+def test_flux_lusgs_is_flux_difference_minus_dissipation():
+    """FluxLUSGS3D(q, dQ, n) = F_n(Q+dQ) - F_n(Q) - (|v.n|+a) dQ, with Q the
+    conservative state of q: checked against an independent conservative
+    evaluation, and zero for dQ = 0 (up to rounding)."""
+    rng = np.random.default_rng(3)
+    gam = 1.4
+    p_bar = derived(0.5, 0.0)["p_bar"]
+    for _ in range(30):
+        q = random_state(rng, p_bar)
+        n = unit(rng.normal(size=3))
+        rho, u, v, w, pg = q
+        Q = np.array([rho, rho * u, rho * v, rho * w, pg / (gam - 1) + 0.5 * rho * (u * u + v * v + w * w)])
+        dQ = rng.normal(size=5) * np.array([0.01, 1, 1, 1, 100])
+        Qn = Q + dQ
+        prim = lambda U: np.array([U[0], U[1] / U[0], U[2] / U[0], U[3] / U[0],  # noqa: E731
+                                   (gam - 1) * (U[4] - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0])])
+        eig = abs(u * n[0] + v * n[1] + w * n[2]) + np.sqrt(gam * (pg + p_bar) / rho)
+        expect = phys_flux(prim(Qn), n, p_bar) - phys_flux(q, n, p_bar) - eig * dQ
+        got = _flux_lusgs(q, dQ, n)
+        scale = np.abs(phys_flux(q, n, p_bar)).max()
+        np.testing.assert_allclose(got, expect, rtol=0, atol=1e-9 * scale)
+        assert np.abs(_flux_lusgs(q, np.zeros(5), n)).max() < 1e-9 * scale
+
+
+def test_geometry_closure_and_volume():
+    """Sum of outward A n over each cell's faces vanishes; the cells fill the
+    box minus the bump."""
+    import minifs_amd as mfs
+
+    g = mfs.Grid.box_bump(6).decouple().preprocess(mfs.make_params(0.5, 0.0, 10.0))
+    info = g.info
+    nC, nB, nF = info["nTCell"], info["nBFace"], info["nTFace"]
+    f2c = g.array("f2c").reshape(-1, 2)
+    A = g.array("area")
+    N = np.stack([g.array("xfn"), g.array("yfn"), g.array("zfn")], 1) * A[:, None]
+    acc = np.zeros((nC, 3))
+    np.add.at(acc, f2c[:, 0], N)
+    np.add.at(acc, f2c[nB:, 1], -N[nB:])
+    assert np.abs(acc).max() < 1e-15
+    vol = g.array("vol")[:nC]
+    assert (vol > 0).all()
+    # z' = z + b(x,y)(1-z): the volume lost is the integral of the bump
+    x = np.linspace(0, 1, 2001)
+    X, Y = np.meshgrid(x, x)
+    lost = np.trapezoid(np.trapezoid(0.05 * np.exp(-((X - .5) ** 2 + (Y - .5) ** 2) / 0.02), x), x)
+    assert abs(vol.sum() - (1 - lost)) < 2e-3
+
+
+def test_freestream_residual_vanishes_away_from_the_wall(tmp_path):
+    """Uniform freestream with consistent far-field ghosts: the residual of a
+    cell with no wall face is rounding only (closure), and the residual sum
+    telescopes to minus the boundary fluxes."""
+    import minifs_amd as mfs
+
+    g = mfs.Grid.box_bump(6).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    o = Oracle(mesh, bc, 0.5, 0.0, 10.0)
+    o.run(1, 0)
+    res = o.get("res").reshape(5, -1)
+    g.preprocess(mfs.make_params(0.5, 0.0, 10.0))
+    nC, nB = g.info["nTCell"], g.info["nBFace"]
+    f2c = g.array("f2c").reshape(-1, 2)
+    wall_cells = set(f2c[:nB][g.array("bctype") == 3, 0].tolist())
+    mask = np.ones(nC, bool)
+    mask[list(wall_cells)] = False
+    p_bar = derived(0.5, 0.0)["p_bar"]
+    scale = p_bar * (1.0 / 6) ** 2
+    assert np.abs(res[:, mask]).max() < 1e-9 * scale
+    assert np.abs(res[:, ~mask]).max() > 1e-6 * scale  # the bump deflects the flow
