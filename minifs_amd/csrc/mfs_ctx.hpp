@@ -28,6 +28,7 @@ struct mfs_ctx {
     hipStream_t stream = nullptr;
     int nC = 0, nF = 0, nB = 0, nN = 0;
     int precision = MFS_FP64;
+    int maxf = 4;  // face slots per cell used by the kernels (4 or 6)
     mfs_params P{};
     mfs::FlowConst K{};
     bool broken = false;
@@ -35,12 +36,16 @@ struct mfs_ctx {
     std::vector<mfs::DevBuf> bufs;  // every allocation, freed at destroy
     size_t device_bytes = 0;
 
-    // state (two buffers, 5 blocks of nC+nB each) and outputs
+    // cell-major state (5 doubles per cell, ghosts at nC+k), two buffers
     double* q[2] = {nullptr, nullptr};
     int cur = 0;
-    double *res = nullptr, *dt = nullptr, *diag = nullptr, *dq = nullptr;
-    double *gx = nullptr, *gy = nullptr, *gz = nullptr, *qn = nullptr;
-    // faces
+    double *res = nullptr;  // 5 per cell
+    double *dt = nullptr, *diag = nullptr;
+    double *dqf = nullptr;  // forward-sweep DQ, 5 per cell
+    double *dq = nullptr;   // final DQ, 5 per cell incl. ghosts (0)
+    double *gx = nullptr, *gy = nullptr, *gz = nullptr;  // reference layout: 5 blocks of nC+nB
+    double* qn = nullptr;   // node values, 5 per device node
+    // faces (device order)
     int2* fcell = nullptr;
     double4* fgeo = nullptr;
     double2* fdn = nullptr;
@@ -55,15 +60,19 @@ struct mfs_ctx {
     int nsym = 0;
     int* sym_node = nullptr;
     double* sym_n = nullptr;
+    std::vector<int> node_of_dev;  // host: device node -> mesh node
+    std::vector<int> cell_of_dev;  // host: device cell -> mesh cell
     // sweeps
-    int *sched_f = nullptr, *sched_b = nullptr, *flags = nullptr;
-    unsigned int* tickets = nullptr;  // [0] forward, [1] backward
+    int *sched_f = nullptr, *sched_b = nullptr, *rep_f = nullptr, *rep_b = nullptr;
+    int* depmask = nullptr;
+    int cell0 = 0;  // device id of mesh cell 0
+    int nchunk_f = 0, nchunk_b = 0;
     int nlev_f = 0, nlev_b = 0;
-    int epoch = 0;
+    int sweep_grid = 0;  // co-resident waves of the cooperative sweep launch
     // reductions and history
     double* part = nullptr;  // nblk * 8: 5 squared-norm partials, dt min, dt max, spare
     double* run = nullptr;   // [0] dt_max [1] dt_min [2] clamp flag [3..7] last norms
-    int* err = nullptr;      // [0] sweep spin limit hit
+    int* err = nullptr;      // [0] sweep wait limit hit
     double* hist = nullptr;
     int hist_cap = 0, hist_n = 0;
     int nblk_cell = 0;

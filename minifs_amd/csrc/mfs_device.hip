@@ -1,16 +1,11 @@
 // HIP kernels and device context of the MI355X solver path (gfx950).
 //
-// Layout: structure-of-arrays in the mesh's (TDTree) numbering.  State q is
-// 5 blocks of nC+nB doubles (rho, u, v, w, gauge p; ghost cell of boundary
-// face k at nC+k), double-buffered so the backward sweep can write the
-// updated state while its neighbours still read the old one.  Faces carry
-// (c1,c2), (nx,ny,nz,area) and the two cell-to-face distances; cells carry
-// their faces in ascending face id.
-//
 // Every reference loop that scatters face contributions into cells is run
 // here as a per-cell gather over the cell's faces in ascending face id,
 // which adds the same terms in the same order as the reference's serial
 // face loop: results are bitwise equal, with no atomics and no colouring.
+// The device layout (cell-major state, first-touch face and node storage,
+// level schedules for the sweeps) is built on the host by mfs_prep.cpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,523 +27,10 @@ namespace dev {
 
 constexpr int kBlock = 256;
 constexpr int kSweepWave = 64;
-constexpr unsigned kSpinLimit = 1u << 22;  // bound on dependency polls of one sweep wave
+constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wave
 
-__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
-
-// Deterministic block sum of NV doubles per thread (fixed shuffle tree,
-// then a fixed LDS combination of the block's waves).
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
-    __shared__ double s[kBlock / 64][NV];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1)
-#pragma unroll
-        for (int j = 0; j < NV; ++j) v[j] += __shfl_down(v[j], off, 64);
-    const int w = threadIdx.x >> 6;
-    if (lane_id() == 0)
-#pragma unroll
-        for (int j = 0; j < NV; ++j) s[w][j] = v[j];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            double a = s[0][j];
-            for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a += s[k][j];
-            out[j] = a;
-        }
-    }
-}
-
-__device__ __forceinline__ void block_minmax(double vmin, double vmax, double* out_min, double* out_max) {
-    __shared__ double smin_[kBlock / 64], smax_[kBlock / 64];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        vmin = smin(vmin, __shfl_down(vmin, off, 64));
-        vmax = smax(vmax, __shfl_down(vmax, off, 64));
-    }
-    const int w = threadIdx.x >> 6;
-    if (lane_id() == 0) {
-        smin_[w] = vmin;
-        smax_[w] = vmax;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = smin_[0], b = smax_[0];
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
-            a = smin(a, smin_[k]);
-            b = smax(b, smax_[k]);
-        }
-        *out_min = a;
-        *out_max = b;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Residual: InviscidFlux + RoeFlux_noprec + LoadFlux (cal_flux.cpp:84-192,
-// 682-833, 1219-1254).  res[c] starts at 0 (after ZeroResiduals) or at its
-// current value (ACCUM) and receives -F for faces where c is c1 and +F where
-// c is c2, in ascending face order.  The block partial sums of res^2 feed
-// the residual norm of main.cpp:294-317.
-template <bool ACCUM>
-__global__ __launch_bounds__(kBlock) void k_flux(int nC, int n, const int* __restrict__ cfo, const int* __restrict__ cf,
-                                                 const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
-                                                 const double* __restrict__ q, double* __restrict__ res,
-                                                 double* __restrict__ part, FlowConst K) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (c < nC) {
-        if (ACCUM)
-#pragma unroll
-            for (int j = 0; j < 5; ++j) r[j] = res[(size_t)j * nC + c];
-        const int k0 = cfo[c], k1 = cfo[c + 1];
-        for (int k = k0; k < k1; ++k) {
-            const int f = cf[k];
-            const int2 cc = fcell[f];
-            const double4 g = fgeo[f];
-            double ql[5], qr[5], F[5];
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                ql[j] = q[(size_t)j * n + cc.x];
-                qr[j] = q[(size_t)j * n + cc.y];
-            }
-            roe_flux<double>(ql, qr, g.x, g.y, g.z, g.w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
-            if (cc.x == c) {
-#pragma unroll
-                for (int j = 0; j < 5; ++j) r[j] -= F[j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < 5; ++j) r[j] += F[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 5; ++j) res[(size_t)j * nC + c] = r[j];
-    }
-    double s[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) s[j] = r[j] * r[j];
-    block_sum<5>(s, part + (size_t)blockIdx.x * 8);
-}
-
-__global__ void k_fill(double* p, size_t n, double v) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// ---------------------------------------------------------------------------
-// Local time step: TimeStepNormal_new + CellIsMG + LimitTimeStep
-// (compute_timestep.cpp:758-823, 1315-1338, 836-918), fused with the
-// LU-SGS diagonal Diag = vol/dt + sum 0.5*A*(|u.n|+a)*lhs_omga
-// (forward_lusgs.cpp:94-99, 554-595) when WITH_DIAG.  Both use the cell's
-// own |u.n| + a on every face, evaluated once per (cell, face).
-template <bool WITH_DIAG>
-__global__ __launch_bounds__(kBlock) void k_timestep(int nC, int n, const int* __restrict__ cfo,
-                                                     const int* __restrict__ cf, const int2* __restrict__ fcell,
-                                                     const double4* __restrict__ fgeo, const double2* __restrict__ fdn,
-                                                     const double* __restrict__ q, const double* __restrict__ vol,
-                                                     double* __restrict__ dt, double* __restrict__ diag,
-                                                     double* __restrict__ part, FlowConst K, double cfl,
-                                                     double cfl_min) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    double vmin = kBIGd, vmax = 0.0;
-    if (c < nC) {
-        const double rho = q[c], u = q[(size_t)n + c], v = q[2 * (size_t)n + c], w = q[3 * (size_t)n + c],
-                     p = q[4 * (size_t)n + c];
-        const double c2tmp = K.gam * (p + K.p_bar) / rho;
-        const double a = sqrt(c2tmp);
-        double dtc = kBIGd;
-        int det = 1;
-        const int k0 = cfo[c], k1 = cfo[c + 1];
-        for (int k = k0; k < k1; ++k) {
-            const int f = cf[k];
-            const int2 cc = fcell[f];
-            const double4 g = fgeo[f];
-            const double2 d = fdn[f];
-            const double dn = (cc.x == c) ? d.x : d.y;
-            double vn = u * g.x + v * g.y + w * g.z;
-            vn = fabs(vn);
-            const double eigv = vn + a;
-            dtc = smin(dtc, dn / eigv);
-            const double p1 = q[4 * (size_t)n + cc.x], p2 = q[4 * (size_t)n + cc.y];
-            const double dp = fabs(p2 - p1) / (p2 + p1 + K.p_bar + K.p_bar);
-            if (dp > 0.1) det = 0;
-        }
-        double cfl_tmp;
-        if (p > K.p_break)
-            cfl_tmp = cfl;
-        else if (p < K.p_min)
-            cfl_tmp = cfl_min;
-        else
-            cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
-        if (!det) cfl_tmp = cfl_min;
-        dtc *= cfl_tmp;
-        dt[c] = dtc;
-        vmin = dtc;
-        vmax = dtc;
-        if (WITH_DIAG) {
-            double dg = vol[c] / dtc;
-            for (int k = k0; k < k1; ++k) {
-                const double4 g = fgeo[cf[k]];
-                double vn = u * g.x + v * g.y + w * g.z;
-                vn = fabs(vn);
-                const double eig = vn + a;
-                dg += 0.5 * g.w * eig * K.lhs_omga;
-            }
-            diag[c] = dg;
-        }
-    }
-    block_minmax(vmin, vmax, part + (size_t)blockIdx.x * 8 + 5, part + (size_t)blockIdx.x * 8 + 6);
-}
-
-// Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
-// Only cells whose dt exceeds *lim are recomputed when lim is given (the
-// ratio_dtmax clamp of compute_timestep.cpp:929-945).
-__global__ __launch_bounds__(kBlock) void k_diag(int nC, int n, const int* __restrict__ cfo, const int* __restrict__ cf,
-                                                 const double4* __restrict__ fgeo, const double* __restrict__ q,
-                                                 const double* __restrict__ vol, double* __restrict__ dt,
-                                                 double* __restrict__ diag, const double* __restrict__ run,
-                                                 double ratio_dtmax, FlowConst K) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= nC) return;
-    double dtc = dt[c];
-    if (run) {
-        if (run[2] == 0.0) return;
-        const double lim = ratio_dtmax * run[1];
-        if (!(dtc > lim)) return;
-        dtc = lim;
-        dt[c] = dtc;
-    }
-    const double rho = q[c], u = q[(size_t)n + c], v = q[2 * (size_t)n + c], w = q[3 * (size_t)n + c],
-                 p = q[4 * (size_t)n + c];
-    const double a = sqrt(K.gam * (p + K.p_bar) / rho);
-    double dg = vol[c] / dtc;
-    for (int k = cfo[c]; k < cfo[c + 1]; ++k) {
-        const double4 g = fgeo[cf[k]];
-        double vn = u * g.x + v * g.y + w * g.z;
-        vn = fabs(vn);
-        dg += 0.5 * g.w * (vn + a) * K.lhs_omga;
-    }
-    diag[c] = dg;
-}
-
-// ---------------------------------------------------------------------------
-// Reductions.  mode bits: 1 = residual norms -> run[3..7], 2 = dt extrema
-// (running max/min of compute_timestep.cpp:914-918, clamp flag), 4 = write
-// history record `slot` from run[], 8 = reset the sweep tickets.
-__global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
-                                                   double* __restrict__ hist, int slot, int mode, double ratio_dtmax,
-                                                   unsigned int* tickets) {
-    if (mode & 1) {
-        double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < nblk; b += kBlock)
-#pragma unroll
-            for (int j = 0; j < 5; ++j) s[j] += part[(size_t)b * 8 + j];
-        __shared__ double tot[5];
-        block_sum<5>(s, tot);
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int j = 0; j < 5; ++j) run[3 + j] = sqrt(tot[j]);
-    }
-    if (mode & 2) {
-        double a = kBIGd, b = 0.0;
-        for (int k = threadIdx.x; k < nblk; k += kBlock) {
-            a = smin(a, part[(size_t)k * 8 + 5]);
-            b = smax(b, part[(size_t)k * 8 + 6]);
-        }
-        __shared__ double mn, mx;
-        block_minmax(a, b, &mn, &mx);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double dmax = smax(run[0], mx);
-            const double dmin = smin(run[1], mn);
-            run[0] = dmax;
-            run[1] = dmin;
-            run[2] = (dmax / dmin > ratio_dtmax) ? 1.0 : 0.0;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (mode & 4) {
-            double* h = hist + (size_t)slot * 7;
-            for (int j = 0; j < 5; ++j) h[j] = run[3 + j];
-            h[5] = run[0];
-            h[6] = run[1];
-        }
-        if (mode & 8) {
-            tickets[0] = 0u;
-            tickets[1] = 0u;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// LU-SGS sweeps (SolveLUSGS3D, forward_lusgs.cpp:803-1249) with
-// LUSGSLayer[i] = i: the forward sweep of cell c uses the final DQ of its
-// lower-numbered face neighbours, the backward sweep that of its
-// higher-numbered neighbours and of its ghost cells (DQ = 0).  Cells are
-// visited in level order (sched); each 64-cell chunk is claimed through a
-// ticket counter, so a wave only waits for chunks claimed (and therefore
-// running) before it, and it publishes each finished cell with a release
-// fence and a per-cell epoch flag.  The arithmetic per cell is the serial
-// sweep's, in the serial face order, so DQ is bitwise equal to it.
-__device__ __forceinline__ int flag_load(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void flag_store(int* p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-struct SweepArgs {
-    int nC, nB, n;
-    const int* cfo;
-    const int* cf;
-    const int2* fcell;
-    const double4* fgeo;
-    const double* q;      // state read by the sweep
-    const double* res;    // forward: initial DQ
-    const double* diag;
-    double* dq;
-    double* qout;         // backward: updated state
-    const int* sched;
-    int* flags;
-    unsigned int* ticket;
-    int* err;
-    int epoch;
-};
-
-template <bool FWD>
-__global__ __launch_bounds__(kSweepWave) void k_sweep(SweepArgs A, FlowConst K) {
-    unsigned t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(A.ticket, 1u);
-    t = __shfl(t, 0, 64);
-    const int pos = (int)t * kSweepWave + (int)threadIdx.x;
-    const bool active = pos < A.nC;
-    const int c = active ? A.sched[pos] : 0;
-    const int n = A.n, nC = A.nC;
-
-    int k0 = 0, k1 = 0;
-    unsigned pend = 0;
-    double qc[5], D[5], dg = 1.0;
-    if (active) {
-        k0 = A.cfo[c];
-        k1 = A.cfo[c + 1];
-        for (int k = k0; k < k1 && k - k0 < 32; ++k) {
-            const int f = A.cf[k];
-            if (f < A.nB) continue;
-            const int2 cc = A.fcell[f];
-            const int nb = (cc.x == c) ? cc.y : cc.x;
-            if (FWD ? (nb < c) : (nb > c)) pend |= 1u << (k - k0);
-        }
-#pragma unroll
-        for (int j = 0; j < 5; ++j) qc[j] = A.q[(size_t)j * n + c];
-        dg = A.diag[c];
-        if (FWD) {
-#pragma unroll
-            for (int j = 0; j < 5; ++j) D[j] = A.res[(size_t)j * nC + c];
-        }
-    }
-    bool done = !active;
-    unsigned spins = 0;
-    const int want = A.epoch;
-    while (true) {
-        if (!done && pend) {
-            unsigned p = pend;
-            while (p) {
-                const int b = __builtin_ctz(p);
-                p &= p - 1;
-                const int f = A.cf[k0 + b];
-                const int2 cc = A.fcell[f];
-                const int nb = (cc.x == c) ? cc.y : cc.x;
-                if (flag_load(A.flags + nb) >= want) pend &= ~(1u << b);
-            }
-        }
-        const bool ready = !done && pend == 0;
-        if (__ballot(ready)) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (ready) {
-                if (!FWD) {
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) D[j] = A.dq[(size_t)j * n + c];
-                }
-                double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                const bool skip_faces = FWD && (c == 0);  // forward_lusgs.cpp:803-804
-                if (!skip_faces) {
-                    for (int k = k0; k < k1; ++k) {
-                        const int f = A.cf[k];
-                        const int2 cc = A.fcell[f];
-                        const int nb = (cc.x == c) ? cc.y : cc.x;
-                        if (FWD ? !(nb < c) : !(nb > c)) continue;
-                        const double4 g = A.fgeo[f];
-                        double nx = g.x, ny = g.y, nz = g.z;
-                        if (cc.y == c) {
-                            nx = -nx;
-                            ny = -ny;
-                            nz = -nz;
-                        }
-                        double qn[5], dqn[5], fl[5];
-#pragma unroll
-                        for (int j = 0; j < 5; ++j) qn[j] = A.q[(size_t)j * n + nb];
-                        if (nb < nC) {
-#pragma unroll
-                            for (int j = 0; j < 5; ++j) dqn[j] = A.dq[(size_t)j * n + nb];
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < 5; ++j) dqn[j] = 0.0;
-                        }
-                        flux_lusgs<double>(qn, dqn, nx, ny, nz, K.gam, K.p_bar, K.lhs_omga, fl);
-                        if (FWD) {
-                            const double tmp = 0.5 * g.w;
-#pragma unroll
-                            for (int j = 0; j < 5; ++j) D[j] -= tmp * fl[j];
-                        } else {
-                            const double tmp = g.w;
-#pragma unroll
-                            for (int j = 0; j < 5; ++j) acc[j] += tmp * fl[j];
-                        }
-                    }
-                }
-                if (FWD) {
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) D[j] /= dg;
-                    if (c != 0) dq_limit<double>(qc, D, K.gamm1, K.rho_min, K.rho_max, K.p_min, K.p_max);
-                } else {
-                    const double tmp = 2.0 * dg;
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) D[j] -= acc[j] / tmp;
-                    dq_limit<double>(qc, D, K.gamm1, K.rho_min, K.rho_max, K.p_min, K.p_max);
-                }
-#pragma unroll
-                for (int j = 0; j < 5; ++j) A.dq[(size_t)j * n + c] = D[j];
-                if (!FWD) {
-                    double qnew[5];
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) qnew[j] = qc[j];
-                    update_cell(qnew, D, K);
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) A.qout[(size_t)j * n + c] = qnew[j];
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            if (ready) {
-                flag_store(A.flags + c, want);
-                done = true;
-            }
-        }
-        if (!__ballot(!done)) break;
-        if (++spins > kSpinLimit) {
-            if (lane_id() == 0) atomicExch(A.err, 1);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Node values (CompNodeVar3D_dist, cal_gradient.cpp:1922-2372): each node
-// sums its weighted contributions in the reference's order, then divides
-// by WeightNode + TINY.
-__global__ __launch_bounds__(kBlock) void k_node(int nN, int n, const int* __restrict__ nlo,
-                                                 const int2* __restrict__ nlab, const double* __restrict__ nlw,
-                                                 const double* __restrict__ nden, const double* __restrict__ q,
-                                                 double* __restrict__ qn) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= nN) return;
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = nlo[p]; k < nlo[p + 1]; ++k) {
-        const int2 ab = nlab[k];
-        const double w = nlw[k];
-        if (ab.y < 0) {
-#pragma unroll
-            for (int j = 0; j < 5; ++j) acc[j] += q[(size_t)j * n + ab.x] * w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                double fq = q[(size_t)j * n + ab.x] + q[(size_t)j * n + ab.y];
-                fq *= 0.5;
-                acc[j] += fq * w;
-            }
-        }
-    }
-    const double d = nden[p];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) qn[(size_t)j * nN + p] = acc[j] / d;
-}
-
-// Symmetry-plane velocity projection of the node values (cal_gradient.cpp:2332-2360).
-__global__ void k_node_symm(int nsym, int nN, const int* __restrict__ sym_node, const double* __restrict__ sym_n,
-                            double* __restrict__ qn) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsym) return;
-    const int p = sym_node[s];
-    const double xs = sym_n[3 * s], ys = sym_n[3 * s + 1], zs = sym_n[3 * s + 2];
-    const double un = qn[(size_t)nN + p], vn = qn[2 * (size_t)nN + p], wn = qn[3 * (size_t)nN + p];
-    const double vv = un * xs + vn * ys + wn * zs;
-    qn[(size_t)nN + p] = un - vv * xs;
-    qn[2 * (size_t)nN + p] = vn - vv * ys;
-    qn[3 * (size_t)nN + p] = wn - vv * zs;
-}
-
-// Green-Gauss gradient (CompGradientQ_Gauss_Node, cal_gradient.cpp:1740-1795,
-// 1889-1897): face values from the node values (interior faces, SYMM and
-// INTERFACE boundary faces) or the face-centre average of cell and ghost.
-__global__ __launch_bounds__(kBlock) void k_grad(int nC, int nB, int n, int nN, const int* __restrict__ cfo,
-                                                 const int* __restrict__ cf, const int2* __restrict__ fcell,
-                                                 const double4* __restrict__ fgeo, const int* __restrict__ fno,
-                                                 const int* __restrict__ fn, const int* __restrict__ btype,
-                                                 const double* __restrict__ q, const double* __restrict__ qn,
-                                                 const double* __restrict__ vol, double* __restrict__ gx,
-                                                 double* __restrict__ gy, double* __restrict__ gz) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= nC) return;
-    double ax[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ay[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, az[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = cfo[c]; k < cfo[c + 1]; ++k) {
-        const int f = cf[k];
-        const int2 cc = fcell[f];
-        const double4 g = fgeo[f];
-        double qs[5];
-        bool node_mean = true;
-        if (f < nB) {
-            const int t = btype[f];
-            node_mean = (t == BC_INTERFACE || t == BC_SYMM);
-        }
-        if (node_mean) {
-            const int j0 = fno[f], j1 = fno[f + 1];
-            const double cnt = (double)(j1 - j0);
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                double s = 0.0;
-                for (int m = j0; m < j1; ++m) s += qn[(size_t)j * nN + fn[m]];
-                s /= cnt;
-                qs[j] = s;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 5; ++j) qs[j] = 0.5 * (q[(size_t)j * n + cc.x] + q[(size_t)j * n + cc.y]);
-        }
-        const bool minus = (f >= nB) && (cc.y == c);
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const double s = qs[j] * g.w;
-            const double tx = s * g.x, ty = s * g.y, tz = s * g.z;
-            if (minus) {
-                ax[j] -= tx;
-                ay[j] -= ty;
-                az[j] -= tz;
-            } else {
-                ax[j] += tx;
-                ay[j] += ty;
-                az[j] += tz;
-            }
-        }
-    }
-    const double vc = vol[c];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-        gx[(size_t)j * n + c] = ax[j] / vc;
-        gy[(size_t)j * n + c] = ay[j] / vc;
-        gz[(size_t)j * n + c] = az[j] / vc;
-    }
-}
+#include "mfs_kernels.hpp"
+#include "mfs_sweep.hpp"
 
 }  // namespace dev
 
@@ -594,11 +76,11 @@ struct Timed {
         if (!ctx->timing) return;
         a = take();
         b = take();
-        if (a) hipEventRecord(a, ctx->stream);
+        if (a) (void)hipEventRecord(a, ctx->stream);
     }
     ~Timed() {
         if (!ctx->timing || !a || !b) return;
-        hipEventRecord(b, ctx->stream);
+        (void)hipEventRecord(b, ctx->stream);
         ctx->pending.push_back({name, {a, b}});
     }
     hipEvent_t take() {
@@ -639,17 +121,17 @@ double cfl_of_iter(const mfs_params& P, int iter_done) {
 bool ensure_hist(mfs_ctx* ctx, int need) {
     if (need <= ctx->hist_cap) return true;
     int cap = std::max(need, std::max(64, 2 * ctx->hist_cap));
-    double* nh = nullptr;
     void* v = nullptr;
     if (!hip_ok(ctx, hipMalloc(&v, sizeof(double) * 7 * (size_t)cap), "hipMalloc(history)")) return false;
-    nh = static_cast<double*>(v);
-    if (ctx->hist && ctx->hist_n > 0)
-        if (!hip_ok(ctx, hipMemcpyAsync(nh, ctx->hist, sizeof(double) * 7 * ctx->hist_n, hipMemcpyDeviceToDevice, ctx->stream),
-                    "copy history"))
-            return false;
+    double* nh = static_cast<double*>(v);
+    if (ctx->hist && ctx->hist_n > 0 &&
+        !hip_ok(ctx,
+                hipMemcpyAsync(nh, ctx->hist, sizeof(double) * 7 * ctx->hist_n, hipMemcpyDeviceToDevice, ctx->stream),
+                "copy history"))
+        return false;
     if (ctx->hist) {
-        hipStreamSynchronize(ctx->stream);
-        hipFree(ctx->hist);
+        if (!hip_ok(ctx, hipStreamSynchronize(ctx->stream), "history sync")) return false;
+        (void)hipFree(ctx->hist);
     }
     ctx->hist = nh;
     ctx->hist_cap = cap;
@@ -657,57 +139,64 @@ bool ensure_hist(mfs_ctx* ctx, int need) {
 }
 
 // ---- launch helpers
-bool launch_flux(mfs_ctx* ctx, bool accum) {
+template <int MAXF>
+bool launch_flux_t(mfs_ctx* ctx, bool accum) {
     Timed t(ctx, "flux");
     const double* q = ctx->q[ctx->cur];
-    const int n = ctx->nC + ctx->nB;
     if (accum)
-        hipLaunchKernelGGL(dev::k_flux<true>, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n, ctx->cfo,
-                           ctx->cf, ctx->fcell, ctx->fgeo, q, ctx->res, ctx->part, ctx->K);
+        hipLaunchKernelGGL((dev::k_flux<true, MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, q, ctx->res, ctx->part, ctx->K);
     else
-        hipLaunchKernelGGL(dev::k_flux<false>, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n, ctx->cfo,
-                           ctx->cf, ctx->fcell, ctx->fgeo, q, ctx->res, ctx->part, ctx->K);
+        hipLaunchKernelGGL((dev::k_flux<false, MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, q, ctx->res, ctx->part, ctx->K);
     return check_launch(ctx, "k_flux");
 }
+bool launch_flux(mfs_ctx* ctx, bool accum) {
+    return ctx->maxf == 4 ? launch_flux_t<4>(ctx, accum) : launch_flux_t<6>(ctx, accum);
+}
 
-bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
+template <int MAXF>
+bool launch_timestep_t(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
     Timed t(ctx, "timestep");
     const double* q = ctx->q[ctx->cur];
-    const int n = ctx->nC + ctx->nB;
     if (with_diag)
-        hipLaunchKernelGGL(dev::k_timestep<true>, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
-                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, q, ctx->vol, ctx->dt, ctx->diag,
-                           ctx->part, ctx->K, cfl, cfl_min);
+        hipLaunchKernelGGL((dev::k_timestep<true, MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, q, ctx->vol, ctx->dt, ctx->diag, ctx->part,
+                           ctx->K, cfl, cfl_min);
     else
-        hipLaunchKernelGGL(dev::k_timestep<false>, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
-                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, q, ctx->vol, ctx->dt, ctx->diag,
-                           ctx->part, ctx->K, cfl, cfl_min);
+        hipLaunchKernelGGL((dev::k_timestep<false, MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, q, ctx->vol, ctx->dt, ctx->diag, ctx->part,
+                           ctx->K, cfl, cfl_min);
     return check_launch(ctx, "k_timestep");
+}
+bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
+    return ctx->maxf == 4 ? launch_timestep_t<4>(ctx, with_diag, cfl, cfl_min)
+                          : launch_timestep_t<6>(ctx, with_diag, cfl, cfl_min);
 }
 
 bool launch_reduce(mfs_ctx* ctx, int mode, int slot) {
     Timed t(ctx, "reduce");
     hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, (int)ctx->nblk_cell, ctx->part, ctx->run,
-                       ctx->hist, slot, mode, ctx->P.ratio_dtmax, ctx->tickets);
+                       ctx->hist, slot, mode, ctx->P.ratio_dtmax);
     return check_launch(ctx, "k_reduce");
 }
 
-bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
-    Timed t(ctx, "diag");
-    const int n = ctx->nC + ctx->nB;
-    hipLaunchKernelGGL(dev::k_diag, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n, ctx->cfo, ctx->cf,
-                       ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt, ctx->diag, clamp_only ? ctx->run : nullptr,
-                       ctx->P.ratio_dtmax, ctx->K);
+template <int MAXF>
+bool launch_diag_t(mfs_ctx* ctx, bool clamp_only) {
+    Timed t(ctx, clamp_only ? "clamp" : "diag");
+    hipLaunchKernelGGL((dev::k_diag<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->cfo,
+                       ctx->cf, ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt, ctx->diag,
+                       clamp_only ? ctx->run : nullptr, ctx->P.ratio_dtmax, ctx->K);
     return check_launch(ctx, "k_diag");
 }
+bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
+    return ctx->maxf == 4 ? launch_diag_t<4>(ctx, clamp_only) : launch_diag_t<6>(ctx, clamp_only);
+}
 
-bool launch_sweeps(mfs_ctx* ctx) {
-    const int n = ctx->nC + ctx->nB;
-    const unsigned nchunk = nblk(ctx->nC, kSweepWave);
+template <int MAXF>
+bool launch_sweeps_t(mfs_ctx* ctx) {
     dev::SweepArgs A;
     A.nC = ctx->nC;
-    A.nB = ctx->nB;
-    A.n = n;
     A.cfo = ctx->cfo;
     A.cf = ctx->cf;
     A.fcell = ctx->fcell;
@@ -715,54 +204,67 @@ bool launch_sweeps(mfs_ctx* ctx) {
     A.q = ctx->q[ctx->cur];
     A.res = ctx->res;
     A.diag = ctx->diag;
-    A.dq = ctx->dq;
+    A.dqf = ctx->dqf;
+    A.dqb = ctx->dq;
     A.qout = ctx->q[1 - ctx->cur];
-    A.flags = ctx->flags;
     A.err = ctx->err;
+    A.depmask = ctx->depmask;
+    A.cell0 = ctx->cell0;
+    const dim3 block(kSweepWave);
     {
         Timed t(ctx, "fwd_sweep");
         A.sched = ctx->sched_f;
-        A.ticket = ctx->tickets;
-        A.epoch = ctx->epoch + 1;
-        hipLaunchKernelGGL(dev::k_sweep<true>, dim3(nchunk), dim3(kSweepWave), 0, ctx->stream, A, ctx->K);
-        if (!check_launch(ctx, "k_sweep<fwd>")) return false;
+        A.rep = ctx->rep_f;
+        A.nchunks = ctx->nchunk_f;
+        const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.nchunks));
+        void* args[] = {&A, &ctx->K};
+        if (!hip_ok(ctx,
+                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<true, MAXF>, grid, block, args, 0, ctx->stream),
+                    "k_sweep<fwd>"))
+            return false;
     }
     {
         Timed t(ctx, "bwd_sweep");
         A.sched = ctx->sched_b;
-        A.ticket = ctx->tickets + 1;
-        A.epoch = ctx->epoch + 2;
-        hipLaunchKernelGGL(dev::k_sweep<false>, dim3(nchunk), dim3(kSweepWave), 0, ctx->stream, A, ctx->K);
-        if (!check_launch(ctx, "k_sweep<bwd>")) return false;
+        A.rep = ctx->rep_b;
+        A.nchunks = ctx->nchunk_b;
+        const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.nchunks));
+        void* args[] = {&A, &ctx->K};
+        if (!hip_ok(ctx,
+                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<false, MAXF>, grid, block, args, 0, ctx->stream),
+                    "k_sweep<bwd>"))
+            return false;
     }
-    ctx->epoch += 2;
     ctx->cur = 1 - ctx->cur;
     return true;
 }
+bool launch_sweeps(mfs_ctx* ctx) { return ctx->maxf == 4 ? launch_sweeps_t<4>(ctx) : launch_sweeps_t<6>(ctx); }
 
-bool launch_gradient(mfs_ctx* ctx) {
+template <int MAXF>
+bool launch_gradient_t(mfs_ctx* ctx) {
     const int n = ctx->nC + ctx->nB;
     const double* q = ctx->q[ctx->cur];
     {
         Timed t(ctx, "node");
-        hipLaunchKernelGGL(dev::k_node, dim3(nblk(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, n, ctx->nlo, ctx->nlab,
+        hipLaunchKernelGGL(dev::k_node, dim3(nblk(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo, ctx->nlab,
                            ctx->nlw, ctx->nden, q, ctx->qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
-            hipLaunchKernelGGL(dev::k_node_symm, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, ctx->stream, ctx->nsym, ctx->nN,
+            hipLaunchKernelGGL(dev::k_node_symm, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, ctx->stream, ctx->nsym,
                                ctx->sym_node, ctx->sym_n, ctx->qn);
             if (!check_launch(ctx, "k_node_symm")) return false;
         }
     }
     {
         Timed t(ctx, "gradient");
-        hipLaunchKernelGGL(dev::k_grad, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->nB, n, ctx->nN,
-                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fno, ctx->fn, ctx->btype, q, ctx->qn, ctx->vol,
-                           ctx->gx, ctx->gy, ctx->gz);
+        hipLaunchKernelGGL((dev::k_grad<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n, ctx->cfo,
+                           ctx->cf, ctx->fcell, ctx->fgeo, ctx->fno, ctx->fn, ctx->btype, q, ctx->qn, ctx->vol, ctx->gx,
+                           ctx->gy, ctx->gz);
         if (!check_launch(ctx, "k_grad")) return false;
     }
     return true;
 }
+bool launch_gradient(mfs_ctx* ctx) { return ctx->maxf == 4 ? launch_gradient_t<4>(ctx) : launch_gradient_t<6>(ctx); }
 
 bool usable(mfs_ctx* ctx) {
     if (!ctx) {
@@ -781,20 +283,30 @@ bool usable(mfs_ctx* ctx) {
 }
 
 void free_all(mfs_ctx* ctx) {
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (auto& b : ctx->bufs) hipFree(b.p);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs) (void)hipFree(b.p);
     ctx->bufs.clear();
-    if (ctx->hist) hipFree(ctx->hist);
+    if (ctx->hist) (void)hipFree(ctx->hist);
     ctx->hist = nullptr;
     for (auto& pe : ctx->pending) {
-        hipEventDestroy(pe.second.first);
-        hipEventDestroy(pe.second.second);
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
     }
     ctx->pending.clear();
-    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     ctx->event_pool.clear();
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     ctx->stream = nullptr;
+}
+
+template <int MAXF>
+int sweep_occupancy(int* per_cu) {
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, MAXF>, kSweepWave, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, MAXF>, kSweepWave, 0) != hipSuccess)
+        return -1;
+    *per_cu = std::min(a, b);
+    return 0;
 }
 
 }  // namespace
@@ -820,22 +332,22 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
         set_error("mfs_ctx_create: " + e);
         return MFS_ERR_MESH;
     }
-    for (int c = 0; c < L.nC; ++c)
-        if (L.cfo[c + 1] - L.cfo[c] > 32) {
-            set_error("mfs_ctx_create: cells with more than 32 faces are not supported");
-            return MFS_ERR_MESH;
-        }
+    if (L.max_faces > dev::kMaxFaces) {
+        set_error("mfs_ctx_create: cells with more than 6 faces are not supported");
+        return MFS_ERR_MESH;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         set_error("mfs_ctx_create: no HIP device available");
         return MFS_ERR_HIP;
     }
     mfs_ctx* ctx = new mfs_ctx();
-    hipGetDevice(&ctx->device);
+    (void)hipGetDevice(&ctx->device);
     ctx->nC = L.nC;
     ctx->nF = L.nF;
     ctx->nB = L.nB;
     ctx->nN = L.nN;
+    ctx->maxf = L.max_faces <= 4 ? 4 : 6;
     ctx->precision = precision;
     ctx->P = *p;
     ctx->K.gam = p->gam;
@@ -853,12 +365,18 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ctx->K.betq = 1.0 + ctx->K.alpq * ctx->K.phiq;
     ctx->nlev_f = L.nlev_f;
     ctx->nlev_b = L.nlev_b;
+    ctx->nchunk_f = (int)(L.sched_f.size() / 64);
+    ctx->nchunk_b = (int)(L.sched_b.size() / 64);
     ctx->nblk_cell = (int)nblk(L.nC);
+    ctx->node_of_dev = L.node_of_dev;
+    ctx->cell_of_dev = L.cell_of_dev;
+    ctx->cell0 = L.dev_cell0;
     const size_t n = (size_t)L.nC + L.nB;
     bool ok = hip_ok(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     ok = ok && dalloc(ctx, ctx->q[0], 5 * n, "q0") && dalloc(ctx, ctx->q[1], 5 * n, "q1");
     ok = ok && dalloc(ctx, ctx->res, 5 * (size_t)L.nC, "res") && dalloc(ctx, ctx->dt, L.nC, "dt") &&
-         dalloc(ctx, ctx->diag, L.nC, "diag") && dalloc(ctx, ctx->dq, 5 * n, "dq");
+         dalloc(ctx, ctx->diag, L.nC, "diag") && dalloc(ctx, ctx->dq, 5 * n, "dq") &&
+         dalloc(ctx, ctx->dqf, 5 * (size_t)L.nC, "dq_forward");
     ok = ok && dalloc(ctx, ctx->gx, 5 * n, "dqdx") && dalloc(ctx, ctx->gy, 5 * n, "dqdy") &&
          dalloc(ctx, ctx->gz, 5 * n, "dqdz") && dalloc(ctx, ctx->qn, 5 * (size_t)L.nN, "qnode");
     ok = ok && dalloc(ctx, ctx->fcell, L.nF, "fcell") && dalloc(ctx, ctx->fgeo, L.nF, "fgeo") &&
@@ -871,11 +389,20 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ctx->nsym = (int)L.sym_node.size();
     ok = ok && dalloc(ctx, ctx->sym_node, L.sym_node.size(), "sym_node") &&
          dalloc(ctx, ctx->sym_n, L.sym_n.size(), "sym_n");
-    ok = ok && dalloc(ctx, ctx->sched_f, L.nC, "sched_f") && dalloc(ctx, ctx->sched_b, L.nC, "sched_b") &&
-         dalloc(ctx, ctx->flags, L.nC, "flags") && dalloc(ctx, ctx->tickets, 2, "tickets");
+    ok = ok && dalloc(ctx, ctx->sched_f, L.sched_f.size(), "sched_f") &&
+         dalloc(ctx, ctx->sched_b, L.sched_b.size(), "sched_b") && dalloc(ctx, ctx->rep_f, L.rep_f.size(), "rep_f") &&
+         dalloc(ctx, ctx->rep_b, L.rep_b.size(), "rep_b") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err");
     ok = ok && ensure_hist(ctx, 64);
+    if (ok) {
+        // co-resident waves for the cooperative sweep launch
+        int per_cu = 0, cus = 0;
+        ok = (ctx->maxf == 4 ? sweep_occupancy<4>(&per_cu) : sweep_occupancy<6>(&per_cu)) == 0 &&
+             hip_ok(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
+        if (!ok) set_error("mfs_ctx_create: occupancy query failed");
+        ctx->sweep_grid = std::max(1, std::min(per_cu, 16) * cus);
+    }
     if (ok) {
         std::vector<int2> fc(L.nF);
         std::vector<double2> fd(L.nF);
@@ -885,7 +412,10 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
         }
         std::vector<int2> nab(L.nla.size());
         for (size_t k = 0; k < L.nla.size(); ++k) nab[k] = make_int2(L.nla[k], L.nlb[k]);
-        ok = upload(ctx, ctx->q[0], L.q.data(), 5 * n, "upload q") && upload(ctx, ctx->q[1], L.q.data(), 5 * n, "upload q") &&
+        std::vector<double> qa(5 * n);
+        for (size_t c = 0; c < n; ++c)
+            for (int j = 0; j < 5; ++j) qa[c * 5 + j] = L.q[(size_t)j * n + c];
+        ok = upload(ctx, ctx->q[0], qa.data(), 5 * n, "upload q") && upload(ctx, ctx->q[1], qa.data(), 5 * n, "upload q") &&
              upload(ctx, ctx->fcell, fc.data(), L.nF, "upload fcell") &&
              upload(ctx, ctx->fgeo, reinterpret_cast<const double4*>(L.fgeo.data()), L.nF, "upload fgeo") &&
              upload(ctx, ctx->fdn, fd.data(), L.nF, "upload fdn") && upload(ctx, ctx->fno, L.fno.data(), L.nF + 1, "upload fno") &&
@@ -898,13 +428,21 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              upload(ctx, ctx->nden, L.nden.data(), L.nN, "upload nden") &&
              upload(ctx, ctx->sym_node, L.sym_node.data(), L.sym_node.size(), "upload sym") &&
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
-             upload(ctx, ctx->sched_f, L.sched_f.data(), L.nC, "upload sched_f") &&
-             upload(ctx, ctx->sched_b, L.sched_b.data(), L.nC, "upload sched_b");
+             upload(ctx, ctx->sched_f, L.sched_f.data(), L.sched_f.size(), "upload sched_f") &&
+             upload(ctx, ctx->sched_b, L.sched_b.data(), L.sched_b.size(), "upload sched_b") &&
+             upload(ctx, ctx->rep_f, L.rep_f.data(), L.rep_f.size(), "upload rep_f") &&
+             upload(ctx, ctx->rep_b, L.rep_b.data(), L.rep_b.size(), "upload rep_b") &&
+             upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask");
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
-        ok = ok && hip_ok(ctx, hipMemsetAsync(ctx->flags, 0, sizeof(int) * L.nC, ctx->stream), "memset flags") &&
-             hip_ok(ctx, hipMemsetAsync(ctx->tickets, 0, sizeof(unsigned) * 2, ctx->stream), "memset tickets") &&
-             hip_ok(ctx, hipMemsetAsync(ctx->err, 0, sizeof(int) * 4, ctx->stream), "memset err") &&
+        if (ok) {
+            double sentinel;
+            std::memcpy(&sentinel, &dev::kSentinel, sizeof(sentinel));
+            const size_t cnt = 5 * (size_t)L.nC;
+            hipLaunchKernelGGL(dev::k_fill, dim3(nblk(cnt)), dim3(kBlock), 0, ctx->stream, ctx->dqf, cnt, sentinel);
+            ok = check_launch(ctx, "k_fill(dq_forward)");
+        }
+        ok = ok && hip_ok(ctx, hipMemsetAsync(ctx->err, 0, sizeof(int) * 4, ctx->stream), "memset err") &&
              hip_ok(ctx, hipMemsetAsync(ctx->dq, 0, sizeof(double) * 5 * n, ctx->stream), "memset dq") &&
              hip_ok(ctx, hipMemsetAsync(ctx->res, 0, sizeof(double) * 5 * L.nC, ctx->stream), "memset res") &&
              hip_ok(ctx, hipMemsetAsync(ctx->dt, 0, sizeof(double) * L.nC, ctx->stream), "memset dt") &&
@@ -927,7 +465,7 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
 
 extern "C" int mfs_ctx_destroy(mfs_ctx* ctx) {
     if (!ctx) return MFS_OK;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     free_all(ctx);
     delete ctx;
     return MFS_OK;
@@ -953,18 +491,13 @@ extern "C" int mfs_compute_timestep(mfs_ctx* ctx) {
     ctx->cfl_min = cfl_min;
     if (!launch_timestep(ctx, false, cfl, cfl_min)) return MFS_ERR_HIP;
     if (!launch_reduce(ctx, 2, -1)) return MFS_ERR_HIP;
-    // ratio_dtmax clamp (compute_timestep.cpp:929-945): dt only
-    Timed t(ctx, "clamp");
-    const int n = ctx->nC + ctx->nB;
-    hipLaunchKernelGGL(dev::k_diag, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n, ctx->cfo, ctx->cf,
-                       ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt, ctx->diag, ctx->run, ctx->P.ratio_dtmax, ctx->K);
-    return check_launch(ctx, "k_diag(clamp)") ? MFS_OK : MFS_ERR_HIP;
+    // ratio_dtmax clamp (compute_timestep.cpp:929-945)
+    return launch_diag(ctx, true) ? MFS_OK : MFS_ERR_HIP;
 }
 
 extern "C" int mfs_forward_lusgs(mfs_ctx* ctx) {
     if (!usable(ctx)) return MFS_ERR_STATE;
     if (!launch_diag(ctx, false)) return MFS_ERR_HIP;
-    if (!launch_reduce(ctx, 8, -1)) return MFS_ERR_HIP;
     return launch_sweeps(ctx) ? MFS_OK : MFS_ERR_HIP;
 }
 
@@ -999,8 +532,8 @@ extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
         if (!launch_flux(ctx, false)) return MFS_ERR_HIP;
         // ComputeTimeStep with the LU-SGS diagonal fused in.
         if (!launch_timestep(ctx, true, cfl, cfl_min)) return MFS_ERR_HIP;
-        if (!launch_reduce(ctx, 1 | 2 | 4 | 8, ctx->hist_n)) return MFS_ERR_HIP;
-        if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // only acts when the ratio_dtmax clamp triggers
+        if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
+        if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
         // ForwardLUSGS: sweeps + update
         if (!launch_sweeps(ctx)) return MFS_ERR_HIP;
         // CalculateGradient
@@ -1016,7 +549,7 @@ extern "C" int mfs_sync(mfs_ctx* ctx) {
         set_error("null context");
         return MFS_ERR_ARG;
     }
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     if (!hip_ok(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")) return MFS_ERR_HIP;
     int err[4] = {0, 0, 0, 0};
     if (!hip_ok(ctx, hipMemcpy(err, ctx->err, sizeof(err), hipMemcpyDeviceToHost), "read error flags")) return MFS_ERR_HIP;
@@ -1026,7 +559,11 @@ extern "C" int mfs_sync(mfs_ctx* ctx) {
         set_error("LU-SGS sweep exceeded its dependency wait limit");
         return MFS_ERR_STATE;
     }
-    return ctx->broken ? MFS_ERR_STATE : MFS_OK;
+    if (ctx->broken) {
+        set_error("context is in an error state after an earlier failure");
+        return MFS_ERR_STATE;
+    }
+    return MFS_OK;
 }
 
 extern "C" int mfs_history_count(mfs_ctx* ctx) { return ctx ? ctx->hist_n : 0; }
@@ -1044,73 +581,115 @@ extern "C" int mfs_get_history(mfs_ctx* ctx, double* out, int max_records) {
     return cnt;
 }
 
-static double* field_ptr(mfs_ctx* ctx, int field, size_t* count, int* state_block) {
+// Field geometry: reference-layout element count and the device array.
+namespace {
+enum class FieldKind { StateBlock, CellMajor5, Scalar, SoA, NodeMajor5 };
+struct FieldRef {
+    double* dev = nullptr;
+    size_t count = 0;   // reference-layout element count
+    size_t rows = 0;    // cells or nodes for cell/node-major arrays
+    FieldKind kind = FieldKind::Scalar;
+    int block = 0;      // state variable for StateBlock
+};
+FieldRef field_ref(mfs_ctx* ctx, int field) {
     const size_t n = (size_t)ctx->nC + ctx->nB;
-    *state_block = -1;
+    FieldRef f;
     switch (field) {
         case MFS_F_RHO:
         case MFS_F_U:
         case MFS_F_V:
         case MFS_F_W:
         case MFS_F_P:
-            *count = n;
-            *state_block = field;
-            return ctx->q[ctx->cur] + (size_t)field * n;
-        case MFS_F_RES: *count = 5 * (size_t)ctx->nC; return ctx->res;
-        case MFS_F_DT: *count = ctx->nC; return ctx->dt;
-        case MFS_F_DQ: *count = 5 * n; return ctx->dq;
-        case MFS_F_DQDX: *count = 5 * n; return ctx->gx;
-        case MFS_F_DQDY: *count = 5 * n; return ctx->gy;
-        case MFS_F_DQDZ: *count = 5 * n; return ctx->gz;
-        case MFS_F_DIAG: *count = ctx->nC; return ctx->diag;
-        case MFS_F_QNODE: *count = 5 * (size_t)ctx->nN; return ctx->qn;
-        default: *count = 0; return nullptr;
+            f = {ctx->q[ctx->cur], n, n, FieldKind::StateBlock, field};
+            break;
+        case MFS_F_RES: f = {ctx->res, 5 * (size_t)ctx->nC, (size_t)ctx->nC, FieldKind::CellMajor5, 0}; break;
+        case MFS_F_DT: f = {ctx->dt, (size_t)ctx->nC, (size_t)ctx->nC, FieldKind::Scalar, 0}; break;
+        case MFS_F_DQ: f = {ctx->dq, 5 * n, n, FieldKind::CellMajor5, 0}; break;
+        case MFS_F_DQDX: f = {ctx->gx, 5 * n, n, FieldKind::SoA, 0}; break;
+        case MFS_F_DQDY: f = {ctx->gy, 5 * n, n, FieldKind::SoA, 0}; break;
+        case MFS_F_DQDZ: f = {ctx->gz, 5 * n, n, FieldKind::SoA, 0}; break;
+        case MFS_F_DIAG: f = {ctx->diag, (size_t)ctx->nC, (size_t)ctx->nC, FieldKind::Scalar, 0}; break;
+        case MFS_F_QNODE: f = {ctx->qn, 5 * (size_t)ctx->nN, (size_t)ctx->nN, FieldKind::NodeMajor5, 0}; break;
+        default: break;
     }
+    return f;
 }
+}  // namespace
 
-extern "C" size_t mfs_field_count(mfs_ctx* ctx, int field) {
-    if (!ctx) return 0;
-    size_t cnt = 0;
-    int sb;
-    field_ptr(ctx, field, &cnt, &sb);
-    return cnt;
-}
+extern "C" size_t mfs_field_count(mfs_ctx* ctx, int field) { return ctx ? field_ref(ctx, field).count : 0; }
 
 extern "C" int mfs_download(mfs_ctx* ctx, int field, double* dst, size_t count) {
     if (!ctx || !dst) {
         set_error("mfs_download: null argument");
         return MFS_ERR_ARG;
     }
-    size_t cnt = 0;
-    int sb;
-    double* src = field_ptr(ctx, field, &cnt, &sb);
-    if (!src || count != cnt) {
+    FieldRef f = field_ref(ctx, field);
+    if (!f.dev || count != f.count) {
         set_error("mfs_download: unknown field or wrong count");
         return MFS_ERR_ARG;
     }
     int r = mfs_sync(ctx);
     if (r != MFS_OK) return r;
-    return hip_ok(ctx, hipMemcpy(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost), "download") ? MFS_OK : MFS_ERR_HIP;
+    std::vector<double> tmp(f.kind == FieldKind::Scalar ? f.rows : 5 * f.rows);
+    if (!hip_ok(ctx, hipMemcpy(tmp.data(), f.dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost), "download"))
+        return MFS_ERR_HIP;
+    // device row -> mesh row: cells through cell_of_dev, ghosts unchanged, nodes through node_of_dev
+    const size_t nC = (size_t)ctx->nC;
+    auto mesh_row = [&](size_t d) -> size_t {
+        if (f.kind == FieldKind::NodeMajor5) return (size_t)ctx->node_of_dev[d];
+        return d < nC ? (size_t)ctx->cell_of_dev[d] : d;
+    };
+    switch (f.kind) {
+        case FieldKind::Scalar:
+            for (size_t d = 0; d < f.rows; ++d) dst[mesh_row(d)] = tmp[d];
+            break;
+        case FieldKind::SoA:
+            for (int j = 0; j < 5; ++j)
+                for (size_t d = 0; d < f.rows; ++d) dst[(size_t)j * f.rows + mesh_row(d)] = tmp[(size_t)j * f.rows + d];
+            break;
+        case FieldKind::StateBlock:
+            for (size_t d = 0; d < f.rows; ++d) dst[mesh_row(d)] = tmp[d * 5 + f.block];
+            break;
+        case FieldKind::CellMajor5:
+        case FieldKind::NodeMajor5:
+            for (size_t d = 0; d < f.rows; ++d)
+                for (int j = 0; j < 5; ++j) dst[(size_t)j * f.rows + mesh_row(d)] = tmp[d * 5 + j];
+            break;
+    }
+    return MFS_OK;
 }
 
 extern "C" int mfs_upload(mfs_ctx* ctx, int field, const double* src, size_t count) {
     if (!usable(ctx) || !src) return MFS_ERR_ARG;
-    size_t cnt = 0;
-    int sb;
-    double* dst = field_ptr(ctx, field, &cnt, &sb);
-    if (!dst || count != cnt) {
-        set_error("mfs_upload: unknown field or wrong count");
+    FieldRef f = field_ref(ctx, field);
+    const bool allowed = f.kind == FieldKind::StateBlock || field == MFS_F_RES || field == MFS_F_DT;
+    if (!f.dev || count != f.count || !allowed) {
+        set_error("mfs_upload: unknown or read-only field, or wrong count");
         return MFS_ERR_ARG;
     }
-    if (!hip_ok(ctx, hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, ctx->stream), "upload"))
-        return MFS_ERR_HIP;
-    if (sb >= 0) {
-        const size_t n = (size_t)ctx->nC + ctx->nB;
-        double* other = ctx->q[1 - ctx->cur] + (size_t)sb * n;
-        if (!hip_ok(ctx, hipMemcpyAsync(other, src, sizeof(double) * cnt, hipMemcpyHostToDevice, ctx->stream), "upload"))
+    if (!hip_ok(ctx, hipStreamSynchronize(ctx->stream), "upload sync")) return MFS_ERR_HIP;
+    const size_t nC = (size_t)ctx->nC;
+    auto mesh_row = [&](size_t d) -> size_t { return d < nC ? (size_t)ctx->cell_of_dev[d] : d; };
+    std::vector<double> tmp(f.kind == FieldKind::Scalar ? f.rows : 5 * f.rows);
+    if (f.kind == FieldKind::Scalar) {
+        for (size_t d = 0; d < f.rows; ++d) tmp[d] = src[mesh_row(d)];
+    } else if (f.kind == FieldKind::StateBlock) {
+        if (!hip_ok(ctx, hipMemcpy(tmp.data(), f.dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost), "upload read"))
             return MFS_ERR_HIP;
+        for (size_t d = 0; d < f.rows; ++d) tmp[d * 5 + f.block] = src[mesh_row(d)];
+        // both state buffers, so the ghost values stay identical in each
+        return hip_ok(ctx, hipMemcpy(ctx->q[0], tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice), "upload") &&
+                       hip_ok(ctx, hipMemcpy(ctx->q[1], tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice),
+                              "upload")
+                   ? MFS_OK
+                   : MFS_ERR_HIP;
+    } else {
+        for (size_t d = 0; d < f.rows; ++d)
+            for (int j = 0; j < 5; ++j) tmp[d * 5 + j] = src[(size_t)j * f.rows + mesh_row(d)];
     }
-    return hip_ok(ctx, hipStreamSynchronize(ctx->stream), "upload sync") ? MFS_OK : MFS_ERR_HIP;
+    return hip_ok(ctx, hipMemcpy(f.dev, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice), "upload")
+               ? MFS_OK
+               : MFS_ERR_HIP;
 }
 
 extern "C" int mfs_get_scalars(mfs_ctx* ctx, mfs_scalars* s) {

@@ -1,0 +1,412 @@
+// Per-cell and per-node gather kernels of the solver path, included by
+// mfs_device.hip inside namespace mfs::dev.
+//
+// Device layout: state q, residual and node values are stored cell-major
+// (5 consecutive doubles per cell or node) so that a gather of one
+// neighbour touches one or two cache lines; faces and nodes are stored in
+// first-touch order of the cell traversal.  Each kernel visits a cell's
+// faces in ascending mesh face id (the per-cell list order), which is the
+// order in which the reference's face loops add into that cell.
+//
+// The MAXF template bounds the faces per cell (4 for tetrahedral meshes, 6
+// otherwise).  Slots beyond a cell's face count are loaded from the cell's
+// first face, so every load of a cell can be issued at once, and their
+// contributions are skipped.
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
+    __shared__ double s[kBlock / 64][NV];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] += __shfl_down(v[j], off, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s[w][j] = v[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double a = s[0][j];
+            for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a += s[k][j];
+            out[j] = a;
+        }
+    }
+}
+
+__device__ __forceinline__ void block_minmax(double vmin, double vmax, double* out_min, double* out_max) {
+    __shared__ double smin_[kBlock / 64], smax_[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        vmin = smin(vmin, __shfl_down(vmin, off, 64));
+        vmax = smax(vmax, __shfl_down(vmax, off, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        smin_[w] = vmin;
+        smax_[w] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = smin_[0], b = smax_[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            a = smin(a, smin_[k]);
+            b = smax(b, smax_[k]);
+        }
+        *out_min = a;
+        *out_max = b;
+    }
+}
+
+__device__ __forceinline__ void load5(const double* __restrict__ p, double v[5]) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) v[j] = p[j];
+}
+
+__global__ void k_fill(double* p, size_t n, double v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Cell-face slots of one cell: device face ids, padded with the first face.
+template <int MAXF>
+__device__ __forceinline__ int cell_faces(const int* __restrict__ cfo, const int* __restrict__ cf, int c,
+                                          int (&fs)[MAXF]) {
+    const int k0 = cfo[c];
+    const int nf = cfo[c + 1] - k0;
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s) fs[s] = cf[k0 + (s < nf ? s : 0)];
+    return nf;
+}
+
+// ---------------------------------------------------------------------------
+// Residual: InviscidFlux + RoeFlux_noprec + LoadFlux (cal_flux.cpp:84-192,
+// 682-833, 1219-1254).  res[c] starts at 0 (after ZeroResiduals) or at its
+// current value (ACCUM) and receives -F for faces where c is c1 and +F where
+// c is c2, in ascending face order.  Block partial sums of res^2 feed the
+// residual norm of main.cpp:294-317.
+template <bool ACCUM, int MAXF>
+__global__ __launch_bounds__(kBlock) void k_flux(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                 const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                                 const double* __restrict__ q, double* __restrict__ res,
+                                                 double* __restrict__ part, FlowConst K) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (c < nC) {
+        if (ACCUM) load5(res + (size_t)c * 5, r);
+        int fs[MAXF];
+        const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
+        int2 cc[MAXF];
+        double4 g[MAXF];
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            cc[s] = fcell[fs[s]];
+            g[s] = fgeo[fs[s]];
+        }
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            double ql[5], qr[5], F[5];
+            load5(q + (size_t)cc[s].x * 5, ql);
+            load5(q + (size_t)cc[s].y * 5, qr);
+            roe_flux<double>(ql, qr, g[s].x, g[s].y, g[s].z, g[s].w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
+            if (s < nf) {
+                if (cc[s].x == c) {
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) r[j] -= F[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) r[j] += F[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) res[(size_t)c * 5 + j] = r[j];
+    }
+    double s2[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
+    block_sum<5>(s2, part + (size_t)blockIdx.x * 8);
+}
+
+// ---------------------------------------------------------------------------
+// Local time step: TimeStepNormal_new + CellIsMG + LimitTimeStep
+// (compute_timestep.cpp:758-823, 1315-1338, 836-918), fused with the
+// LU-SGS diagonal Diag = vol/dt + sum 0.5*A*(|u.n|+a)*lhs_omga
+// (forward_lusgs.cpp:94-99, 554-595) when WITH_DIAG.  Both use the cell's
+// own |u.n| + a on every face.
+template <bool WITH_DIAG, int MAXF>
+__global__ __launch_bounds__(kBlock) void k_timestep(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                     const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                                     const double2* __restrict__ fdn, const double* __restrict__ q,
+                                                     const double* __restrict__ vol, double* __restrict__ dt,
+                                                     double* __restrict__ diag, double* __restrict__ part, FlowConst K,
+                                                     double cfl, double cfl_min) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double vmin = kBIGd, vmax = 0.0;
+    if (c < nC) {
+        double qc[5];
+        load5(q + (size_t)c * 5, qc);
+        const double rho = qc[0], u = qc[1], v = qc[2], w = qc[3], p = qc[4];
+        int fs[MAXF];
+        const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
+        int2 cc[MAXF];
+        double4 g[MAXF];
+        double2 d[MAXF];
+        double pa[MAXF], pb[MAXF];
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            cc[s] = fcell[fs[s]];
+            g[s] = fgeo[fs[s]];
+            d[s] = fdn[fs[s]];
+        }
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            pa[s] = q[(size_t)cc[s].x * 5 + 4];
+            pb[s] = q[(size_t)cc[s].y * 5 + 4];
+        }
+        const double c2tmp = K.gam * (p + K.p_bar) / rho;
+        const double a = sqrt(c2tmp);
+        double dtc = kBIGd;
+        int det = 1;
+        double eig[MAXF];
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            const double dn = (cc[s].x == c) ? d[s].x : d[s].y;
+            double vn = u * g[s].x + v * g[s].y + w * g[s].z;
+            vn = fabs(vn);
+            eig[s] = vn + a;
+            const double dp = fabs(pb[s] - pa[s]) / (pb[s] + pa[s] + K.p_bar + K.p_bar);
+            if (s < nf) {
+                dtc = smin(dtc, dn / eig[s]);
+                if (dp > 0.1) det = 0;
+            }
+        }
+        double cfl_tmp;
+        if (p > K.p_break)
+            cfl_tmp = cfl;
+        else if (p < K.p_min)
+            cfl_tmp = cfl_min;
+        else
+            cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
+        if (!det) cfl_tmp = cfl_min;
+        dtc *= cfl_tmp;
+        dt[c] = dtc;
+        vmin = dtc;
+        vmax = dtc;
+        if (WITH_DIAG) {
+            double dg = vol[c] / dtc;
+#pragma unroll
+            for (int s = 0; s < MAXF; ++s)
+                if (s < nf) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
+            diag[c] = dg;
+        }
+    }
+    block_minmax(vmin, vmax, part + (size_t)blockIdx.x * 8 + 5, part + (size_t)blockIdx.x * 8 + 6);
+}
+
+// Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
+// With `run`, only cells whose dt exceeds ratio_dtmax*dt_min are clamped
+// and recomputed (compute_timestep.cpp:929-945), and only when the
+// reduction found the ratio exceeded.
+template <int MAXF>
+__global__ __launch_bounds__(kBlock) void k_diag(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                 const double4* __restrict__ fgeo, const double* __restrict__ q,
+                                                 const double* __restrict__ vol, double* __restrict__ dt,
+                                                 double* __restrict__ diag, const double* __restrict__ run,
+                                                 double ratio_dtmax, FlowConst K) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nC) return;
+    double dtc = dt[c];
+    if (run) {
+        if (run[2] == 0.0) return;
+        const double lim = ratio_dtmax * run[1];
+        if (!(dtc > lim)) return;
+        dtc = lim;
+        dt[c] = dtc;
+    }
+    double qc[5];
+    load5(q + (size_t)c * 5, qc);
+    int fs[MAXF];
+    const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
+    const double a = sqrt(K.gam * (qc[4] + K.p_bar) / qc[0]);
+    double dg = vol[c] / dtc;
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s) {
+        const double4 g = fgeo[fs[s]];
+        double vn = qc[1] * g.x + qc[2] * g.y + qc[3] * g.z;
+        vn = fabs(vn);
+        if (s < nf) dg += 0.5 * g.w * (vn + a) * K.lhs_omga;
+    }
+    diag[c] = dg;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  mode bits: 1 = residual norms -> run[3..7], 2 = dt extrema
+// (running max/min of compute_timestep.cpp:914-918, clamp flag), 4 = write
+// history record `slot` from run[].
+__global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
+                                                   double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
+    if (mode & 1) {
+        double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < nblk; b += kBlock)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) s[j] += part[(size_t)b * 8 + j];
+        __shared__ double tot[5];
+        block_sum<5>(s, tot);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int j = 0; j < 5; ++j) run[3 + j] = sqrt(tot[j]);
+    }
+    if (mode & 2) {
+        double a = kBIGd, b = 0.0;
+        for (int k = threadIdx.x; k < nblk; k += kBlock) {
+            a = smin(a, part[(size_t)k * 8 + 5]);
+            b = smax(b, part[(size_t)k * 8 + 6]);
+        }
+        __shared__ double mn, mx;
+        block_minmax(a, b, &mn, &mx);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double dmax = smax(run[0], mx);
+            const double dmin = smin(run[1], mn);
+            run[0] = dmax;
+            run[1] = dmin;
+            run[2] = (dmax / dmin > ratio_dtmax) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (mode & 4)) {
+        double* h = hist + (size_t)slot * 7;
+        for (int j = 0; j < 5; ++j) h[j] = run[3 + j];
+        h[5] = run[0];
+        h[6] = run[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Node values (CompNodeVar3D_dist, cal_gradient.cpp:1922-2372): each node
+// sums its weighted contributions in the reference's order, then divides by
+// WeightNode + TINY.
+__global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
+                                                 const double* __restrict__ nlw, const double* __restrict__ nden,
+                                                 const double* __restrict__ q, double* __restrict__ qn) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= nN) return;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int e0 = nlo[p], e1 = nlo[p + 1];
+    for (int k = e0; k < e1; ++k) {
+        const int2 ab = nlab[k];
+        const double w = nlw[k];
+        double qa[5];
+        load5(q + (size_t)ab.x * 5, qa);
+        if (ab.y < 0) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) acc[j] += qa[j] * w;
+        } else {
+            double qb[5];
+            load5(q + (size_t)ab.y * 5, qb);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                double fq = qa[j] + qb[j];
+                fq *= 0.5;
+                acc[j] += fq * w;
+            }
+        }
+    }
+    const double d = nden[p];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) qn[(size_t)p * 5 + j] = acc[j] / d;
+}
+
+// Symmetry-plane velocity projection of the node values (cal_gradient.cpp:2332-2360).
+__global__ void k_node_symm(int nsym, const int* __restrict__ sym_node, const double* __restrict__ sym_n,
+                            double* __restrict__ qn) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsym) return;
+    const size_t p = (size_t)sym_node[s] * 5;
+    const double xs = sym_n[3 * s], ys = sym_n[3 * s + 1], zs = sym_n[3 * s + 2];
+    const double un = qn[p + 1], vn = qn[p + 2], wn = qn[p + 3];
+    const double vv = un * xs + vn * ys + wn * zs;
+    qn[p + 1] = un - vv * xs;
+    qn[p + 2] = vn - vv * ys;
+    qn[p + 3] = wn - vv * zs;
+}
+
+// Green-Gauss gradient (CompGradientQ_Gauss_Node, cal_gradient.cpp:1740-1795,
+// 1889-1897): face values from the node values (interior faces, SYMM and
+// INTERFACE boundary faces) or the face-centre average of cell and ghost;
+// outputs in the reference's per-variable blocks of nC+nB.
+template <int MAXF>
+__global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __restrict__ cfo,
+                                                 const int* __restrict__ cf, const int2* __restrict__ fcell,
+                                                 const double4* __restrict__ fgeo, const int* __restrict__ fno,
+                                                 const int* __restrict__ fn, const int* __restrict__ btype,
+                                                 const double* __restrict__ q, const double* __restrict__ qn,
+                                                 const double* __restrict__ vol, double* __restrict__ gx,
+                                                 double* __restrict__ gy, double* __restrict__ gz) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nC) return;
+    double ax[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ay[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, az[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int fs[MAXF];
+    const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
+    int2 cc[MAXF];
+    double4 g[MAXF];
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s) {
+        cc[s] = fcell[fs[s]];
+        g[s] = fgeo[fs[s]];
+    }
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s) {
+        if (s >= nf) break;
+        const int f = fs[s];
+        double qs[5];
+        bool node_mean = true;
+        if (cc[s].y >= nC) {
+            const int t = btype[cc[s].y - nC];
+            node_mean = (t == BC_INTERFACE || t == BC_SYMM);
+        }
+        if (node_mean) {
+            const int j0 = fno[f], j1 = fno[f + 1];
+            const double cnt = (double)(j1 - j0);
+            double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int m = j0; m < j1; ++m) {
+                double v5[5];
+                load5(qn + (size_t)fn[m] * 5, v5);
+#pragma unroll
+                for (int j = 0; j < 5; ++j) sum[j] += v5[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) qs[j] = sum[j] / cnt;
+        } else {
+            double qa[5], qb[5];
+            load5(q + (size_t)cc[s].x * 5, qa);
+            load5(q + (size_t)cc[s].y * 5, qb);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) qs[j] = 0.5 * (qa[j] + qb[j]);
+        }
+        const bool minus = (cc[s].y < nC) && (cc[s].y == c);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const double sa = qs[j] * g[s].w;
+            const double tx = sa * g[s].x, ty = sa * g[s].y, tz = sa * g[s].z;
+            if (minus) {
+                ax[j] -= tx;
+                ay[j] -= ty;
+                az[j] -= tz;
+            } else {
+                ax[j] += tx;
+                ay[j] += ty;
+                az[j] += tz;
+            }
+        }
+    }
+    const double vc = vol[c];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        gx[(size_t)j * n + c] = ax[j] / vc;
+        gy[(size_t)j * n + c] = ay[j] / vc;
+        gz[(size_t)j * n + c] = az[j] / vc;
+    }
+}

@@ -27,13 +27,15 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
     L.nB = nB;
     L.nN = nN;
 
-    // ---- faces
-    L.c1.resize(nF);
-    L.c2.resize(nF);
-    L.fgeo.resize(4 * (size_t)nF);
-    L.dn1.resize(nF);
-    L.dn2.assign(nF, 0.0);
-    L.btype.assign(v.bc_type, v.bc_type + nB);
+    // ---- mesh-order face tables and validation
+    std::vector<int> mfo(nF + 1);
+    mfo[0] = 0;
+    for (int i = 0; i < nF; ++i) {
+        if (v.nNPF[i] < 3) return "face with fewer than 3 nodes";
+        mfo[i + 1] = mfo[i] + v.nNPF[i];
+    }
+    for (int k = 0; k < mfo[nF]; ++k)
+        if (v.f2n[k] < 0 || v.f2n[k] >= nN) return "f2n has an invalid node";
     for (int i = 0; i < nF; ++i) {
         const int a = v.f2c[2 * i], b = v.f2c[2 * i + 1];
         if (a < 0 || a >= nC) return "f2c has an invalid first cell";
@@ -42,67 +44,198 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         } else if (b < 0 || b >= nC || b == a) {
             return "interior face has an invalid second cell";
         }
-        L.c1[i] = a;
-        L.c2[i] = b;
-        L.fgeo[4 * (size_t)i + 0] = v.xfn[i];
-        L.fgeo[4 * (size_t)i + 1] = v.yfn[i];
-        L.fgeo[4 * (size_t)i + 2] = v.zfn[i];
-        L.fgeo[4 * (size_t)i + 3] = v.area[i];
-        // TimeStepNormal_new (compute_timestep.cpp:758-823): the same
-        // expression for every iteration, so it is evaluated once here.
-        L.dn1[i] = fabs((v.xfc[i] - v.xcc[a]) * v.xfn[i] + (v.yfc[i] - v.ycc[a]) * v.yfn[i] +
-                        (v.zfc[i] - v.zcc[a]) * v.zfn[i]);
-        if (i >= nB)
-            L.dn2[i] = fabs((v.xfc[i] - v.xcc[b]) * v.xfn[i] + (v.yfc[i] - v.ycc[b]) * v.yfn[i] +
-                            (v.zfc[i] - v.zcc[b]) * v.zfn[i]);
     }
-    L.fno.resize(nF + 1);
-    L.fno[0] = 0;
-    for (int i = 0; i < nF; ++i) {
-        if (v.nNPF[i] < 3) return "face with fewer than 3 nodes";
-        L.fno[i + 1] = L.fno[i] + v.nNPF[i];
-    }
-    L.fn.assign(v.f2n, v.f2n + L.fno[nF]);
-    for (int k = 0; k < L.fno[nF]; ++k)
-        if (L.fn[k] < 0 || L.fn[k] >= nN) return "f2n has an invalid node";
+    L.btype.assign(v.bc_type, v.bc_type + nB);
 
-    // ---- cell -> faces, ascending (CalnFPC/CalC2F)
-    L.cfo.assign(nC + 1, 0);
+    // ---- mesh cell -> faces in ascending mesh face id (CalnFPC/CalC2F)
+    std::vector<int> mcfo(nC + 1, 0), mcf;
     for (int i = 0; i < nF; ++i) {
-        L.cfo[L.c1[i] + 1]++;
-        if (i >= nB) L.cfo[L.c2[i] + 1]++;
+        mcfo[v.f2c[2 * i] + 1]++;
+        if (i >= nB) mcfo[v.f2c[2 * i + 1] + 1]++;
     }
-    for (int c = 0; c < nC; ++c) L.cfo[c + 1] += L.cfo[c];
-    L.cf.resize(L.cfo[nC]);
+    for (int c = 0; c < nC; ++c) mcfo[c + 1] += mcfo[c];
+    mcf.resize(mcfo[nC]);
     {
-        std::vector<int> fill(L.cfo.begin(), L.cfo.end() - 1);
+        std::vector<int> fill(mcfo.begin(), mcfo.end() - 1);
         for (int i = 0; i < nF; ++i) {
-            L.cf[fill[L.c1[i]]++] = i;
-            if (i >= nB) L.cf[fill[L.c2[i]]++] = i;
+            mcf[fill[v.f2c[2 * i]]++] = i;
+            if (i >= nB) mcf[fill[v.f2c[2 * i + 1]]++] = i;
         }
     }
-    for (int c = 0; c < nC; ++c)
-        if (L.cfo[c + 1] == L.cfo[c]) return "cell without faces";
-    L.vol.assign(v.vol, v.vol + nC);
+    L.max_faces = 0;
+    for (int c = 0; c < nC; ++c) {
+        const int k = mcfo[c + 1] - mcfo[c];
+        if (k == 0) return "cell without faces";
+        L.max_faces = std::max(L.max_faces, k);
+    }
+    if (L.max_faces > 8) return "cells with more than 8 faces are not supported";
+    auto other = [&](int c, int f) { return v.f2c[2 * f] == c ? v.f2c[2 * f + 1] : v.f2c[2 * f]; };
+
+    // ---- LU-SGS levels on mesh ids.  With LUSGSLayer[i] = i
+    // (ReorderCellforLUSGS_0, grid_polyhedra.cpp:1127-1142) the forward sweep
+    // of cell c reads the final DQ of every face neighbour with a smaller id
+    // and the backward sweep that of every neighbour with a larger id.
+    L.flevel.assign(nC, 0);
+    L.blevel.assign(nC, 0);
+    int mf = 0, mb = 0;
+    for (int c = 0; c < nC; ++c) {
+        int l = 0;
+        for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+            const int f = mcf[k];
+            if (f < nB) continue;
+            const int nb = other(c, f);
+            if (nb < c) l = std::max(l, L.flevel[nb] + 1);
+        }
+        L.flevel[c] = l;
+        mf = std::max(mf, l);
+    }
+    for (int c = nC - 1; c >= 0; --c) {
+        int l = 0;
+        for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+            const int f = mcf[k];
+            if (f < nB) continue;
+            const int nb = other(c, f);
+            if (nb > c) l = std::max(l, L.blevel[nb] + 1);
+        }
+        L.blevel[c] = l;
+        mb = std::max(mb, l);
+    }
+    L.nlev_f = mf + 1;
+    L.nlev_b = mb + 1;
+
+    // ---- device cell order: forward level, then mesh id
+    std::vector<int> dcell(nC);
+    {
+        std::vector<int> cnt(L.nlev_f + 1, 0);
+        for (int c = 0; c < nC; ++c) cnt[L.flevel[c] + 1]++;
+        for (int l = 0; l < L.nlev_f; ++l) cnt[l + 1] += cnt[l];
+        L.cell_of_dev.resize(nC);
+        for (int c = 0; c < nC; ++c) {
+            const int d = cnt[L.flevel[c]]++;
+            dcell[c] = d;
+            L.cell_of_dev[d] = c;
+        }
+    }
+    L.dev_cell0 = dcell[0];
+    auto dmap = [&](int c) { return c < nC ? dcell[c] : c; };  // ghosts keep nC+k
+
+    // ---- device face order: first touch in the device cell traversal
+    std::vector<int> dface(nF, -1);
+    L.face_of_dev.resize(nF);
+    {
+        int next = 0;
+        for (int d = 0; d < nC; ++d) {
+            const int c = L.cell_of_dev[d];
+            for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+                const int f = mcf[k];
+                if (dface[f] < 0) {
+                    dface[f] = next;
+                    L.face_of_dev[next] = f;
+                    ++next;
+                }
+            }
+        }
+        if (next != nF) return "face not attached to any cell";
+    }
+    // per device cell: device face ids, still in ascending mesh face id,
+    // and the sweep dependency masks
+    L.cfo.assign(nC + 1, 0);
+    L.cf.resize(mcf.size());
+    L.depmask.assign(nC, 0);
+    {
+        int pos = 0;
+        for (int d = 0; d < nC; ++d) {
+            const int c = L.cell_of_dev[d];
+            int s = 0, mask = 0;
+            for (int k = mcfo[c]; k < mcfo[c + 1]; ++k, ++s) {
+                const int f = mcf[k];
+                L.cf[pos++] = dface[f];
+                const int nb = other(c, f);
+                if (nb < c) mask |= 1 << s;  // ghosts have nb >= nC > c
+                if (nb > c) {
+                    mask |= 1 << (8 + s);
+                    if (nb < nC) mask |= 1 << (16 + s);
+                }
+            }
+            L.depmask[d] = mask;
+            L.cfo[d + 1] = pos;
+        }
+    }
+
+    // ---- device node order: first touch over the device faces
+    std::vector<int> dnode(nN, -1);
+    L.node_of_dev.assign(nN, -1);
+    {
+        int next = 0;
+        for (int d = 0; d < nF; ++d) {
+            const int f = L.face_of_dev[d];
+            for (int j = mfo[f]; j < mfo[f + 1]; ++j) {
+                const int p = v.f2n[j];
+                if (dnode[p] < 0) {
+                    dnode[p] = next;
+                    L.node_of_dev[next] = p;
+                    ++next;
+                }
+            }
+        }
+        for (int p = 0; p < nN; ++p)  // nodes used by no face keep their relative order at the end
+            if (dnode[p] < 0) {
+                dnode[p] = next;
+                L.node_of_dev[next] = p;
+                ++next;
+            }
+    }
+
+    // ---- faces in device order
+    L.c1.resize(nF);
+    L.c2.resize(nF);
+    L.fgeo.resize(4 * (size_t)nF);
+    L.dn1.resize(nF);
+    L.dn2.assign(nF, 0.0);
+    L.fno.resize(nF + 1);
+    L.fno[0] = 0;
+    L.fn.clear();
+    L.fn.reserve(mfo[nF]);
+    for (int d = 0; d < nF; ++d) {
+        const int i = L.face_of_dev[d];
+        const int a = v.f2c[2 * i], b = v.f2c[2 * i + 1];
+        L.c1[d] = dmap(a);
+        L.c2[d] = dmap(b);
+        L.fgeo[4 * (size_t)d + 0] = v.xfn[i];
+        L.fgeo[4 * (size_t)d + 1] = v.yfn[i];
+        L.fgeo[4 * (size_t)d + 2] = v.zfn[i];
+        L.fgeo[4 * (size_t)d + 3] = v.area[i];
+        // TimeStepNormal_new (compute_timestep.cpp:758-823): the same
+        // expression in every iteration, so it is evaluated once here.
+        L.dn1[d] = fabs((v.xfc[i] - v.xcc[a]) * v.xfn[i] + (v.yfc[i] - v.ycc[a]) * v.yfn[i] +
+                        (v.zfc[i] - v.zcc[a]) * v.zfn[i]);
+        if (i >= nB)
+            L.dn2[d] = fabs((v.xfc[i] - v.xcc[b]) * v.xfn[i] + (v.yfc[i] - v.ycc[b]) * v.yfn[i] +
+                            (v.zfc[i] - v.zcc[b]) * v.zfn[i]);
+        for (int j = mfo[i]; j < mfo[i + 1]; ++j) L.fn.push_back(dnode[v.f2n[j]]);
+        L.fno[d + 1] = (int)L.fn.size();
+    }
+    L.vol.resize(nC);
+    for (int d = 0; d < nC; ++d) L.vol[d] = v.vol[L.cell_of_dev[d]];
 
     // ---- node weights (ComputeWeight3D_Node, grid_polyhedra_auxiliary_functions.cpp:831-1049)
-    // and the accumulation order of CompNodeVar3D_dist (cal_gradient.cpp:1922-2372).
+    // and the accumulation order of CompNodeVar3D_dist (cal_gradient.cpp:1922-2372),
+    // computed on mesh ids.
     std::vector<double> wnode(nN, 0.0);
-    std::vector<int>& Nmark = L.nmark;
-    Nmark.assign(nN, 0);
-    std::vector<double> wbf(L.fno[nF > nB ? nB : nF], 0.0);  // WeightNodeBFace2C, boundary faces only
+    std::vector<int> Nmark(nN, 0);
+    std::vector<double> wbf(mfo[nB] + 1, 0.0);  // WeightNodeBFace2C
     auto bf_weight = [&](int i, int j, int p1) {
         const double dx = v.x[p1] - v.xfc[i], dy = v.y[p1] - v.yfc[i], dz = v.z[p1] - v.zfc[i];
         double wr = dx * dx + dy * dy + dz * dz;
         wr = sqrt(wr);
         wr = 1. / wr;
-        wbf[L.fno[i] + j] = wr;
+        wbf[mfo[i] + j] = wr;
         wnode[p1] += wr;
     };
     for (int i = 0; i < nB; i++) {
         if (L.btype[i] != BC_WALL) continue;
         for (int j = 0; j < v.nNPF[i]; j++) {
-            const int p1 = L.fn[L.fno[i] + j];
+            const int p1 = v.f2n[mfo[i] + j];
             bf_weight(i, j, p1);
             Nmark[p1] = BC_WALL;
         }
@@ -110,7 +243,7 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
     for (int i = 0; i < nB; i++) {
         if (L.btype[i] != BC_FAR_FIELD && L.btype[i] != BC_FAR_FIELD_MOD) continue;
         for (int j = 0; j < v.nNPF[i]; j++) {
-            const int p1 = L.fn[L.fno[i] + j];
+            const int p1 = v.f2n[mfo[i] + j];
             if (Nmark[p1] == BC_WALL) continue;
             bf_weight(i, j, p1);
             Nmark[p1] = BC_FAR_FIELD;
@@ -120,74 +253,87 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         const int t = L.btype[i];
         if (t == BC_WALL || t == BC_SYMM || t == BC_FAR_FIELD || t == BC_FAR_FIELD_MOD || t == BC_INTERFACE) continue;
         for (int j = 0; j < v.nNPF[i]; j++) {
-            const int p1 = L.fn[L.fno[i] + j];
+            const int p1 = v.f2n[mfo[i] + j];
             if (Nmark[p1] == BC_WALL || Nmark[p1] == BC_FAR_FIELD) continue;
             bf_weight(i, j, p1);
             Nmark[p1] = 101;
         }
     }
-    // cell -> node list of each cell: the nodes of its faces, unique.  Each
-    // (cell,node) pair is visited once by the reference's cell loop, whose
-    // per-node summation order is therefore ascending cell id.
-    std::vector<std::vector<int>> c2n(nC);
-    for (int c = 0; c < nC; ++c) {
-        for (int k = L.cfo[c]; k < L.cfo[c + 1]; ++k) {
-            const int f = L.cf[k];
-            for (int j = L.fno[f]; j < L.fno[f + 1]; ++j) c2n[c].push_back(L.fn[j]);
-        }
-        std::sort(c2n[c].begin(), c2n[c].end());
-        c2n[c].erase(std::unique(c2n[c].begin(), c2n[c].end()), c2n[c].end());
-    }
-    // entries grouped by node, in reference order
-    struct Ent {
-        int a, b;
-        double w;
-    };
-    std::vector<std::vector<Ent>> ent(nN);
-    auto add_face_entries = [&](int i, bool (*skip)(int)) {
-        for (int j = 0; j < v.nNPF[i]; j++) {
-            const int p1 = L.fn[L.fno[i] + j];
-            if (skip(Nmark[p1])) continue;
-            ent[p1].push_back({L.c1[i], L.c2[i], wbf[L.fno[i] + j]});
-        }
-    };
-    for (int i = 0; i < nB; i++)
-        if (L.btype[i] == BC_WALL) add_face_entries(i, [](int) { return false; });
-    for (int i = 0; i < nB; i++)
-        if (L.btype[i] == BC_FAR_FIELD) add_face_entries(i, [](int m) { return m == BC_WALL; });
-    for (int i = 0; i < nB; i++) {
+    // Face loops of CompNodeVar3D_dist: WALL faces, FAR_FIELD faces (not on
+    // WALL nodes), other types (not on WALL or FAR_FIELD nodes); then the
+    // cell loop, whose per-node order is ascending mesh cell id because
+    // each (cell,node) pair is visited once.
+    auto loop_of = [&](int i) {
         const int t = L.btype[i];
-        if (t == BC_WALL || t == BC_SYMM || t == BC_FAR_FIELD || t == BC_INTERFACE) continue;
-        add_face_entries(i, [](int m) { return m == BC_WALL || m == BC_FAR_FIELD; });
-    }
-    for (int c = 0; c < nC; ++c)
-        for (int p1 : c2n[c]) {
-            if (Nmark[p1] != 0) continue;
-            const double dx = v.x[p1] - v.xcc[c], dy = v.y[p1] - v.ycc[c], dz = v.z[p1] - v.zcc[c];
-            double wr = dx * dx + dy * dy + dz * dz;
-            wr = sqrt(wr);
-            wr = 1. / wr;
-            wnode[p1] += wr;
-            ent[p1].push_back({c, -1, wr});
+        if (t == BC_WALL) return 1;
+        if (t == BC_FAR_FIELD) return 2;
+        if (t == BC_SYMM || t == BC_INTERFACE) return 0;
+        return 3;
+    };
+    auto face_entry_ok = [&](int p1, int loop) {
+        if (loop == 1) return true;
+        if (loop == 2) return Nmark[p1] != BC_WALL;
+        return Nmark[p1] != BC_WALL && Nmark[p1] != BC_FAR_FIELD;
+    };
+    std::vector<int> ecount(nN, 0);
+    for (int loop = 1; loop <= 3; ++loop)
+        for (int i = 0; i < nB; i++) {
+            if (loop_of(i) != loop) continue;
+            for (int j = mfo[i]; j < mfo[i + 1]; j++)
+                if (face_entry_ok(v.f2n[j], loop)) ecount[v.f2n[j]]++;
         }
-    std::vector<std::vector<int>>().swap(c2n);
+    std::vector<int> cstamp(nN, -1);
+    for (int c = 0; c < nC; ++c)
+        for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+            const int f = mcf[k];
+            for (int j = mfo[f]; j < mfo[f + 1]; ++j) {
+                const int p1 = v.f2n[j];
+                if (cstamp[p1] == c) continue;
+                cstamp[p1] = c;
+                if (Nmark[p1] == 0) ecount[p1]++;
+            }
+        }
     L.nlo.assign(nN + 1, 0);
-    for (int p = 0; p < nN; ++p) L.nlo[p + 1] = L.nlo[p] + (int)ent[p].size();
+    for (int d = 0; d < nN; ++d) L.nlo[d + 1] = L.nlo[d] + ecount[L.node_of_dev[d]];
     L.nla.resize(L.nlo[nN]);
     L.nlb.resize(L.nlo[nN]);
     L.nlw.resize(L.nlo[nN]);
-    for (int p = 0; p < nN; ++p) {
-        int k = L.nlo[p];
-        for (const Ent& e : ent[p]) {
-            L.nla[k] = e.a;
-            L.nlb[k] = e.b;
-            L.nlw[k] = e.w;
-            ++k;
+    std::vector<int> efill(nN);
+    for (int d = 0; d < nN; ++d) efill[L.node_of_dev[d]] = L.nlo[d];
+    for (int loop = 1; loop <= 3; ++loop)
+        for (int i = 0; i < nB; i++) {
+            if (loop_of(i) != loop) continue;
+            for (int j = 0; j < v.nNPF[i]; j++) {
+                const int p1 = v.f2n[mfo[i] + j];
+                if (!face_entry_ok(p1, loop)) continue;
+                const int e = efill[p1]++;
+                L.nla[e] = dmap(v.f2c[2 * i]);
+                L.nlb[e] = v.f2c[2 * i + 1];  // ghost
+                L.nlw[e] = wbf[mfo[i] + j];
+            }
         }
-    }
-    std::vector<std::vector<Ent>>().swap(ent);
+    std::fill(cstamp.begin(), cstamp.end(), -1);
+    for (int c = 0; c < nC; ++c)
+        for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+            const int f = mcf[k];
+            for (int j = mfo[f]; j < mfo[f + 1]; ++j) {
+                const int p1 = v.f2n[j];
+                if (cstamp[p1] == c) continue;
+                cstamp[p1] = c;
+                if (Nmark[p1] != 0) continue;
+                const double dx = v.x[p1] - v.xcc[c], dy = v.y[p1] - v.ycc[c], dz = v.z[p1] - v.zcc[c];
+                double wr = dx * dx + dy * dy + dz * dz;
+                wr = sqrt(wr);
+                wr = 1. / wr;
+                wnode[p1] += wr;
+                const int e = efill[p1]++;
+                L.nla[e] = dcell[c];
+                L.nlb[e] = -1;
+                L.nlw[e] = wr;
+            }
+        }
     L.nden.resize(nN);
-    for (int p = 0; p < nN; ++p) L.nden[p] = wnode[p] + kTINY_;
+    for (int d = 0; d < nN; ++d) L.nden[d] = wnode[L.node_of_dev[d]] + kTINY_;
 
     // FindNodeSYMM (grid_polyhedra_auxiliary_functions.cpp:2060-2140)
     {
@@ -195,8 +341,8 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         std::vector<double> xs(nN, 0.0), ys(nN, 0.0), zs(nN, 0.0), nface(nN, 0.0);
         for (int i = 0; i < nB; i++) {
             if (L.btype[i] != BC_SYMM) continue;
-            for (int j = L.fno[i]; j < L.fno[i + 1]; j++) {
-                const int p1 = L.fn[j];
+            for (int j = mfo[i]; j < mfo[i + 1]; j++) {
+                const int p1 = v.f2n[j];
                 nodesymm[p1] = 1;
                 xs[p1] += v.xfn[i];
                 ys[p1] += v.yfn[i];
@@ -206,64 +352,65 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         }
         L.sym_node.clear();
         L.sym_n.clear();
-        for (int p = 0; p < nN; ++p)
+        for (int d = 0; d < nN; ++d) {
+            const int p = L.node_of_dev[d];
             if (nodesymm[p] == 1) {
-                L.sym_node.push_back(p);
+                L.sym_node.push_back(d);
                 L.sym_n.push_back(xs[p] / nface[p]);
                 L.sym_n.push_back(ys[p] / nface[p]);
                 L.sym_n.push_back(zs[p] / nface[p]);
             }
+        }
     }
 
-    // ---- LU-SGS level schedule.  With LUSGSLayer[i] = i
-    // (ReorderCellforLUSGS_0, grid_polyhedra.cpp:1127-1142) the forward sweep
-    // of cell c reads the final DQ of every face neighbour with a smaller id
-    // and the backward sweep that of every neighbour with a larger id.
-    L.flevel.assign(nC, 0);
-    L.blevel.assign(nC, 0);
-    int mf = 0, mb = 0;
-    for (int c = 0; c < nC; ++c) {
-        int l = 0;
-        for (int k = L.cfo[c]; k < L.cfo[c + 1]; ++k) {
-            const int f = L.cf[k];
-            if (f < nB) continue;
-            const int nb = (L.c1[f] == c) ? L.c2[f] : L.c1[f];
-            if (nb < c) l = std::max(l, L.flevel[nb] + 1);
-        }
-        L.flevel[c] = l;
-        mf = std::max(mf, l);
-    }
-    for (int c = nC - 1; c >= 0; --c) {
-        int l = 0;
-        for (int k = L.cfo[c]; k < L.cfo[c + 1]; ++k) {
-            const int f = L.cf[k];
-            if (f < nB) continue;
-            const int nb = (L.c1[f] == c) ? L.c2[f] : L.c1[f];
-            if (nb > c) l = std::max(l, L.blevel[nb] + 1);
-        }
-        L.blevel[c] = l;
-        mb = std::max(mb, l);
-    }
-    L.nlev_f = mf + 1;
-    L.nlev_b = mb + 1;
-    auto bucket = [nC](const std::vector<int>& lev, int nlev, bool descending_id, std::vector<int>& out) {
-        std::vector<int> cnt(nlev + 1, 0);
-        for (int c = 0; c < nC; ++c) cnt[lev[c] + 1]++;
-        for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
-        out.resize(nC);
-        if (!descending_id) {
-            for (int c = 0; c < nC; ++c) out[cnt[lev[c]]++] = c;
+    // ---- padded level schedules (device ids) and the latest dependency of each chunk
+    auto build_sched = [&](const std::vector<int>& lev, int nlev, bool fwd, std::vector<int>& sched,
+                           std::vector<int>& rep) {
+        std::vector<int> cnt(nlev, 0);
+        for (int c = 0; c < nC; ++c) cnt[lev[c]]++;
+        std::vector<int> start(nlev + 1, 0);
+        for (int l = 0; l < nlev; ++l) start[l + 1] = start[l] + (cnt[l] + 63) / 64 * 64;
+        sched.assign(start[nlev], -1);
+        std::vector<int> pos(nC);
+        std::vector<int> fillp(start.begin(), start.end() - 1);
+        if (fwd) {
+            for (int c = 0; c < nC; ++c) pos[c] = fillp[lev[c]]++;
         } else {
-            for (int c = nC - 1; c >= 0; --c) out[cnt[lev[c]]++] = c;
+            for (int c = nC - 1; c >= 0; --c) pos[c] = fillp[lev[c]]++;
         }
+        for (int c = 0; c < nC; ++c) sched[pos[c]] = dcell[c];
+        const int nchunk = start[nlev] / 64;
+        rep.assign(nchunk, -1);
+        std::vector<int> repPos(nchunk, -1);
+        for (int c = 0; c < nC; ++c) {
+            const int ch = pos[c] / 64;
+            for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+                const int f = mcf[k];
+                if (f < nB) continue;
+                const int nb = other(c, f);
+                if (fwd ? (nb < c) : (nb > c)) {
+                    if (pos[nb] / 64 >= ch) return std::string("sweep schedule is not topological");
+                    if (pos[nb] > repPos[ch]) {
+                        repPos[ch] = pos[nb];
+                        rep[ch] = dcell[nb];
+                    }
+                }
+            }
+        }
+        return std::string();
     };
-    bucket(L.flevel, L.nlev_f, false, L.sched_f);
-    bucket(L.blevel, L.nlev_b, true, L.sched_b);
+    std::string e = build_sched(L.flevel, L.nlev_f, true, L.sched_f, L.rep_f);
+    if (!e.empty()) return e;
+    e = build_sched(L.blevel, L.nlev_b, false, L.sched_b, L.rep_b);
+    if (!e.empty()) return e;
 
-    // ---- state
+    // ---- state, device order (ghosts unchanged)
     L.q.resize(5 * (size_t)n);
     const double* src[5] = {v.rho, v.u, v.v, v.w, v.p};
-    for (int j = 0; j < 5; ++j) std::copy(src[j], src[j] + n, L.q.begin() + (size_t)j * n);
+    for (int j = 0; j < 5; ++j) {
+        for (int d = 0; d < nC; ++d) L.q[(size_t)j * n + d] = src[j][L.cell_of_dev[d]];
+        for (int k = nC; k < n; ++k) L.q[(size_t)j * n + k] = src[j][k];
+    }
     return "";
 }
 

@@ -6,6 +6,15 @@
 // contributions in exactly the order the reference's serial loops add them;
 // a gather that walks these lists in order performs the same floating-point
 // additions in the same order as the reference's scatter.
+//
+// Storage order on the device is free as long as that order of operations
+// is kept.  Cells are stored in the order of the forward LU-SGS schedule
+// (DAG level, then mesh id), so the 64 cells of a sweep chunk are
+// contiguous; faces and nodes are stored in first-touch order of that cell
+// traversal.  Everything that depends on mesh numbering (per-cell face
+// order, the sweep direction test "neighbour id < cell id", the per-node
+// summation order) is evaluated here on mesh ids and stored as lists or
+// masks.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -17,29 +26,44 @@ namespace mfs {
 
 struct HostLayout {
     int nC = 0, nF = 0, nB = 0, nN = 0;
-    // faces
-    std::vector<int> c1, c2;                   // nF (c2 = nC+k for boundary face k)
+    // cells in device order; cell_of_dev[d] = mesh cell id (ghost k stays at nC+k)
+    std::vector<int> cell_of_dev;
+    int dev_cell0 = 0;                         // device id of mesh cell 0
+    // per device cell: bits 0-7 forward dependency slots (interior neighbour
+    // with a smaller mesh id), bits 8-15 backward slots (neighbour with a
+    // larger mesh id, ghosts included), bits 16-23 backward slots whose
+    // neighbour is an interior cell
+    std::vector<int> depmask;
+    // faces, in device order; face_of_dev[d] = mesh face id
+    std::vector<int> face_of_dev;
+    std::vector<int> c1, c2;                   // nF, device cell ids (c2 = nC+k for boundary face k)
     std::vector<double> fgeo;                  // 4*nF: xfn, yfn, zfn, area
     std::vector<double> dn1, dn2;              // nF: |(x_fc - x_cc) . n| for each side
-    std::vector<int> fno, fn;                  // face->node CSR (F2N)
-    std::vector<int> btype;                    // nB
-    // cells: faces in ascending face id (CalC2F, grid_polyhedra_auxiliary_functions.cpp:1345-1399)
+    std::vector<int> fno, fn;                  // face->node CSR in device node ids
+    std::vector<int> btype;                    // nB, by boundary face (ghost) index k
+    // cells: device face ids in ascending mesh face id (CalC2F,
+    // grid_polyhedra_auxiliary_functions.cpp:1345-1399)
     std::vector<int> cfo, cf;                  // nC+1, sum nFPC
     std::vector<double> vol;                   // nC
-    // node values (CompNodeVar3D_dist): per node, ordered entries
+    int max_faces = 0;
+    // nodes, in device order; node_of_dev[d] = mesh node id
+    std::vector<int> node_of_dev;
+    // node values (CompNodeVar3D_dist): per device node, ordered entries
     // value = q[a] (b < 0) or 0.5*(q[a]+q[b]) (b >= 0, boundary face average)
     std::vector<int> nlo;                      // nN+1
     std::vector<int> nla, nlb;                 // entries
     std::vector<double> nlw;                   // entry weights
     std::vector<double> nden;                  // nN: WeightNode + TINY
-    std::vector<int> nmark;                    // nN (0, WALL, FAR_FIELD, 101)
-    std::vector<int> sym_node;                 // nodes with nodesymm == 1
+    std::vector<int> sym_node;                 // device nodes with nodesymm == 1
     std::vector<double> sym_n;                 // 3 per symmetry node (xfn_n_symm, ...)
-    // LU-SGS sweep schedule
-    std::vector<int> flevel, blevel;           // nC: forward / backward DAG level
-    std::vector<int> sched_f, sched_b;         // cells sorted by (level, id)
+    // LU-SGS sweep schedule: device cell ids by DAG level (mesh-id levels),
+    // each level padded with -1 to a multiple of 64 so that no 64-cell
+    // chunk spans two levels.
+    std::vector<int> flevel, blevel;           // nC, by mesh id
+    std::vector<int> sched_f, sched_b;         // padded schedules
+    std::vector<int> rep_f, rep_b;             // per chunk: dependency published last in schedule order (-1: none)
     int nlev_f = 0, nlev_b = 0;
-    // initial state, 5 blocks of nC+nB
+    // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
 };
 
