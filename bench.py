@@ -37,6 +37,9 @@ KERNEL_BYTES = {
     "flux": (40, 80, 40, 0),
     # c1,c2 + normal/area + two distances per face; q, vol read, dt, Diag write; ghost p
     "timestep": (56, 64, 8, 0),
+    # flux and time step in one pass (mfs_step): face tables once, q + vol read,
+    # res + dt + Diag written, ghost q
+    "flux_ts": (56, 104, 40, 0),
     # forward sweep: faces, q, res, Diag read, DQ write, ghost q
     "fwd_sweep": (40, 144, 40, 0),
     # backward sweep + update: as the forward sweep plus q write (K5 + K6)
@@ -117,6 +120,24 @@ def cpu_port_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: flo
     }
 
 
+def copy_calibration(mib: int, reps: int) -> int:
+    """A plain device-to-device copy of known size, run under
+    `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE): the counters' reading per
+    copied byte is the scale factor that turns the solver kernels' readings
+    into HBM bytes."""
+    import torch
+
+    n = mib * 1024 * 1024 // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                                           "launches": reps}}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,7 +152,11 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--cpu-port-iters", type=int, default=60)
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration, 5)
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",

@@ -91,6 +91,15 @@ typedef struct mfs_grid_info {
  * bump_height*exp(-r^2/(2 bump_sigma^2)); patch 1 WALL, patch 2 FAR_FIELD). */
 int mfs_grid_generate_box_bump(int nx, int ny, int nz, double jitter, double bump_height, double bump_sigma,
                                uint64_t seed, mfs_grid** out);
+/* The same box with another cell kind: MFS_CELLS_TET (as above), MFS_CELLS_HEX
+ * (one hexahedron per hex, 6 quadrilateral faces) or MFS_CELLS_PRISM (two
+ * triangular prisms per hex).  The reference reads any polyhedral FlowStar
+ * mesh (load_grid.cpp:129-273); these exercise its non-tetrahedral paths. */
+#define MFS_CELLS_TET 0
+#define MFS_CELLS_HEX 1
+#define MFS_CELLS_PRISM 2
+int mfs_grid_generate_box(int nx, int ny, int nz, int cells, double jitter, double bump_height, double bump_sigma,
+                          uint64_t seed, mfs_grid** out);
 /* LoadGrid with explicit paths (load_grid.cpp:82-314: FlowStar binary + bc.para). */
 int mfs_grid_load(const char* mesh_path, const char* bc_path, mfs_grid** out);
 int mfs_grid_save(const mfs_grid* g, const char* mesh_path, const char* bc_path);
@@ -192,7 +201,8 @@ int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
 /* Kernel time accounting: when enabled, every launch of the named kernel
  * group is bracketed with HIP events on the context stream. */
 int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
-/* names: "flux","timestep","reduce","fwd_sweep","bwd_sweep","node","gradient","total";
+/* names: "flux","timestep","flux_ts" (both fused, mfs_step),"reduce","diag","clamp",
+ * "fwd_sweep","bwd_sweep","node","gradient";
  * returns accumulated milliseconds and launch count. */
 int mfs_ctx_get_timing(mfs_ctx* ctx, const char* name, double* ms, int* launches);
 int mfs_ctx_reset_timing(mfs_ctx* ctx);

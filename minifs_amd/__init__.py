@@ -50,12 +50,15 @@ class Grid:
 
     @classmethod
     def box_bump(cls, nx: int, ny: int | None = None, nz: int | None = None, jitter: float = 0.1,
-                 bump_height: float = 0.05, bump_sigma: float = 0.1, seed: int = 20260101) -> "Grid":
+                 bump_height: float = 0.05, bump_sigma: float = 0.1, seed: int = 20260101,
+                 cells: str = "tet") -> "Grid":
+        """Box + Gaussian-bump mesh; cells = "tet" (6 per hex), "hex" or "prism" (2 per hex)."""
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
+        kind = {"tet": 0, "hex": 1, "prism": 2}[cells]
         h = C.c_void_p()
-        check(lib().mfs_grid_generate_box_bump(nx, ny, nz, jitter, bump_height, bump_sigma, seed, C.byref(h)),
-              "mfs_grid_generate_box_bump")
+        check(lib().mfs_grid_generate_box(nx, ny, nz, kind, jitter, bump_height, bump_sigma, seed, C.byref(h)),
+              "mfs_grid_generate_box")
         return cls(h)
 
     @classmethod

@@ -66,6 +66,8 @@ _SIGS = {
     "mfs_last_error": (C.c_char_p, []),
     "mfs_grid_generate_box_bump": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                              C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mfs_grid_generate_box": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64, C.POINTER(C.c_void_p)]),
     "mfs_grid_load": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "mfs_grid_save": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "mfs_grid_decouple": (C.c_int, [C.c_void_p]),

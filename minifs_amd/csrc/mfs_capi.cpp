@@ -41,15 +41,22 @@ extern "C" int mfs_params_derive(mfs_params* p) {
 
 extern "C" int mfs_grid_generate_box_bump(int nx, int ny, int nz, double jitter, double bump_height, double bump_sigma,
                                           uint64_t seed, mfs_grid** out) {
-    if (!out || nx < 1 || ny < 1 || nz < 1 || !(jitter >= 0.0 && jitter < 0.25) || !(bump_sigma > 0.0)) {
-        set_error("mfs_grid_generate_box_bump: bad argument (n >= 1, 0 <= jitter < 0.25, sigma > 0)");
+    return mfs_grid_generate_box(nx, ny, nz, MFS_CELLS_TET, jitter, bump_height, bump_sigma, seed, out);
+}
+
+extern "C" int mfs_grid_generate_box(int nx, int ny, int nz, int cells, double jitter, double bump_height,
+                                     double bump_sigma, uint64_t seed, mfs_grid** out) {
+    if (!out || nx < 1 || ny < 1 || nz < 1 || !(jitter >= 0.0 && jitter < 0.25) || !(bump_sigma > 0.0) ||
+        (cells != MFS_CELLS_TET && cells != MFS_CELLS_HEX && cells != MFS_CELLS_PRISM)) {
+        set_error("mfs_grid_generate_box: bad argument (n >= 1, 0 <= jitter < 0.25, sigma > 0, known cell kind)");
         return MFS_ERR_ARG;
     }
     if ((double)nx * ny * nz * 6.0 > 2.0e9) {
-        set_error("mfs_grid_generate_box_bump: mesh too large for 32-bit indices");
+        set_error("mfs_grid_generate_box: mesh too large for 32-bit indices");
         return MFS_ERR_ARG;
     }
     BoxBumpSpec s;
+    s.cells = cells;
     s.nx = nx;
     s.ny = ny;
     s.nz = nz;

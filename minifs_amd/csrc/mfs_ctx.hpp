@@ -63,11 +63,13 @@ struct mfs_ctx {
     std::vector<int> node_of_dev;  // host: device node -> mesh node
     std::vector<int> cell_of_dev;  // host: device cell -> mesh cell
     // sweeps
-    int *sched_f = nullptr, *sched_b = nullptr, *rep_f = nullptr, *rep_b = nullptr;
+    int2* chunk = nullptr;  // per chunk: first device cell, cell count
+    int *rep_f = nullptr, *rep_b = nullptr;
+    int* cfs = nullptr;     // lpc face slots per device cell
     int* depmask = nullptr;
     int cell0 = 0;  // device id of mesh cell 0
-    int nchunk_f = 0, nchunk_b = 0;
-    int nlev_f = 0, nlev_b = 0;
+    int nchunk = 0, lpc = 4;
+    int nlev = 0;
     int sweep_grid = 0;  // co-resident waves of the cooperative sweep launch
     // reductions and history
     double* part = nullptr;  // nblk * 8: 5 squared-norm partials, dt min, dt max, spare

@@ -26,6 +26,7 @@ namespace mfs {
 namespace dev {
 
 constexpr int kBlock = 256;
+constexpr int kXcd = 8;  // XCDs of an MI355X; cell-kernel grids are a multiple of this
 constexpr int kSweepWave = 64;
 constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wave
 
@@ -66,6 +67,8 @@ bool upload(mfs_ctx* ctx, T* dst, const T* src, size_t count, const char* what) 
 }
 
 inline unsigned nblk(size_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+// grid of a kernel that uses xcd_block(): whole multiples of kXcd blocks
+inline unsigned nblk_xcd(size_t n) { return (nblk(n) + dev::kXcd - 1) / dev::kXcd * dev::kXcd; }
 
 // Event bracketing of one launch group when timing is enabled.
 struct Timed {
@@ -174,6 +177,18 @@ bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
                           : launch_timestep_t<6>(ctx, with_diag, cfl, cfl_min);
 }
 
+template <int MAXF>
+bool launch_flux_ts_t(mfs_ctx* ctx, double cfl, double cfl_min) {
+    Timed t(ctx, "flux_ts");
+    hipLaunchKernelGGL((dev::k_flux_ts<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->cfo,
+                       ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res, ctx->dt,
+                       ctx->diag, ctx->part, ctx->K, cfl, cfl_min);
+    return check_launch(ctx, "k_flux_ts");
+}
+bool launch_flux_ts(mfs_ctx* ctx, double cfl, double cfl_min) {
+    return ctx->maxf == 4 ? launch_flux_ts_t<4>(ctx, cfl, cfl_min) : launch_flux_ts_t<6>(ctx, cfl, cfl_min);
+}
+
 bool launch_reduce(mfs_ctx* ctx, int mode, int slot) {
     Timed t(ctx, "reduce");
     hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, (int)ctx->nblk_cell, ctx->part, ctx->run,
@@ -193,12 +208,13 @@ bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
     return ctx->maxf == 4 ? launch_diag_t<4>(ctx, clamp_only) : launch_diag_t<6>(ctx, clamp_only);
 }
 
-template <int MAXF>
+template <int LPC>
 bool launch_sweeps_t(mfs_ctx* ctx) {
     dev::SweepArgs A;
     A.nC = ctx->nC;
-    A.cfo = ctx->cfo;
-    A.cf = ctx->cf;
+    A.nchunks = ctx->nchunk;
+    A.chunk = ctx->chunk;
+    A.cfs = ctx->cfs;
     A.fcell = ctx->fcell;
     A.fgeo = ctx->fgeo;
     A.q = ctx->q[ctx->cur];
@@ -211,34 +227,29 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     A.depmask = ctx->depmask;
     A.cell0 = ctx->cell0;
     const dim3 block(kSweepWave);
+    const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.nchunks));
     {
         Timed t(ctx, "fwd_sweep");
-        A.sched = ctx->sched_f;
         A.rep = ctx->rep_f;
-        A.nchunks = ctx->nchunk_f;
-        const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.nchunks));
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<true, MAXF>, grid, block, args, 0, ctx->stream),
+                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<true, LPC>, grid, block, args, 0, ctx->stream),
                     "k_sweep<fwd>"))
             return false;
     }
     {
         Timed t(ctx, "bwd_sweep");
-        A.sched = ctx->sched_b;
         A.rep = ctx->rep_b;
-        A.nchunks = ctx->nchunk_b;
-        const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.nchunks));
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<false, MAXF>, grid, block, args, 0, ctx->stream),
+                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<false, LPC>, grid, block, args, 0, ctx->stream),
                     "k_sweep<bwd>"))
             return false;
     }
     ctx->cur = 1 - ctx->cur;
     return true;
 }
-bool launch_sweeps(mfs_ctx* ctx) { return ctx->maxf == 4 ? launch_sweeps_t<4>(ctx) : launch_sweeps_t<6>(ctx); }
+bool launch_sweeps(mfs_ctx* ctx) { return ctx->lpc == 4 ? launch_sweeps_t<4>(ctx) : launch_sweeps_t<8>(ctx); }
 
 template <int MAXF>
 bool launch_gradient_t(mfs_ctx* ctx) {
@@ -246,7 +257,7 @@ bool launch_gradient_t(mfs_ctx* ctx) {
     const double* q = ctx->q[ctx->cur];
     {
         Timed t(ctx, "node");
-        hipLaunchKernelGGL(dev::k_node, dim3(nblk(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo, ctx->nlab,
+        hipLaunchKernelGGL(dev::k_node, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo, ctx->nlab,
                            ctx->nlw, ctx->nden, q, ctx->qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
@@ -299,11 +310,11 @@ void free_all(mfs_ctx* ctx) {
     ctx->stream = nullptr;
 }
 
-template <int MAXF>
+template <int LPC>
 int sweep_occupancy(int* per_cu) {
     int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, MAXF>, kSweepWave, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, MAXF>, kSweepWave, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, LPC>, kSweepWave, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, LPC>, kSweepWave, 0) != hipSuccess)
         return -1;
     *per_cu = std::min(a, b);
     return 0;
@@ -363,11 +374,10 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ctx->K.alpq = -0.2;
     ctx->K.phiq = 1. / 0.5;
     ctx->K.betq = 1.0 + ctx->K.alpq * ctx->K.phiq;
-    ctx->nlev_f = L.nlev_f;
-    ctx->nlev_b = L.nlev_b;
-    ctx->nchunk_f = (int)(L.sched_f.size() / 64);
-    ctx->nchunk_b = (int)(L.sched_b.size() / 64);
-    ctx->nblk_cell = (int)nblk(L.nC);
+    ctx->nlev = L.nlev;
+    ctx->nchunk = (int)(L.chunk.size() / 2);
+    ctx->lpc = L.lpc;
+    ctx->nblk_cell = (int)nblk_xcd(L.nC);
     ctx->node_of_dev = L.node_of_dev;
     ctx->cell_of_dev = L.cell_of_dev;
     ctx->cell0 = L.dev_cell0;
@@ -389,8 +399,8 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ctx->nsym = (int)L.sym_node.size();
     ok = ok && dalloc(ctx, ctx->sym_node, L.sym_node.size(), "sym_node") &&
          dalloc(ctx, ctx->sym_n, L.sym_n.size(), "sym_n");
-    ok = ok && dalloc(ctx, ctx->sched_f, L.sched_f.size(), "sched_f") &&
-         dalloc(ctx, ctx->sched_b, L.sched_b.size(), "sched_b") && dalloc(ctx, ctx->rep_f, L.rep_f.size(), "rep_f") &&
+    ok = ok && dalloc(ctx, ctx->chunk, (size_t)ctx->nchunk, "chunk") && dalloc(ctx, ctx->cfs, L.cfs.size(), "cfs") &&
+         dalloc(ctx, ctx->rep_f, L.rep_f.size(), "rep_f") &&
          dalloc(ctx, ctx->rep_b, L.rep_b.size(), "rep_b") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err");
@@ -398,7 +408,7 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     if (ok) {
         // co-resident waves for the cooperative sweep launch
         int per_cu = 0, cus = 0;
-        ok = (ctx->maxf == 4 ? sweep_occupancy<4>(&per_cu) : sweep_occupancy<6>(&per_cu)) == 0 &&
+        ok = (ctx->lpc == 4 ? sweep_occupancy<4>(&per_cu) : sweep_occupancy<8>(&per_cu)) == 0 &&
              hip_ok(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
         if (!ok) set_error("mfs_ctx_create: occupancy query failed");
         ctx->sweep_grid = std::max(1, std::min(per_cu, 16) * cus);
@@ -428,8 +438,8 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              upload(ctx, ctx->nden, L.nden.data(), L.nN, "upload nden") &&
              upload(ctx, ctx->sym_node, L.sym_node.data(), L.sym_node.size(), "upload sym") &&
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
-             upload(ctx, ctx->sched_f, L.sched_f.data(), L.sched_f.size(), "upload sched_f") &&
-             upload(ctx, ctx->sched_b, L.sched_b.data(), L.sched_b.size(), "upload sched_b") &&
+             upload(ctx, ctx->chunk, reinterpret_cast<const int2*>(L.chunk.data()), (size_t)ctx->nchunk, "upload chunk") &&
+             upload(ctx, ctx->cfs, L.cfs.data(), L.cfs.size(), "upload cfs") &&
              upload(ctx, ctx->rep_f, L.rep_f.data(), L.rep_f.size(), "upload rep_f") &&
              upload(ctx, ctx->rep_b, L.rep_b.data(), L.rep_b.size(), "upload rep_b") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask");
@@ -528,10 +538,9 @@ extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
         const double cfl = cfl_of_iter(ctx->P, ctx->iter_done);
         const double cfl_min = 0.5 * cfl;
         ctx->cfl_min = cfl_min;
-        // ZeroResiduals + InviscidFlux: the flux kernel writes res from zero.
-        if (!launch_flux(ctx, false)) return MFS_ERR_HIP;
-        // ComputeTimeStep with the LU-SGS diagonal fused in.
-        if (!launch_timestep(ctx, true, cfl, cfl_min)) return MFS_ERR_HIP;
+        // ZeroResiduals + InviscidFlux + ComputeTimeStep with the LU-SGS
+        // diagonal, in one pass over the faces (res written from zero).
+        if (!launch_flux_ts(ctx, cfl, cfl_min)) return MFS_ERR_HIP;
         if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
         if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
         // ForwardLUSGS: sweeps + update
@@ -718,7 +727,7 @@ extern "C" int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info) {
     info->nBFace = ctx->nB;
     info->nTNode = ctx->nN;
     info->precision = ctx->precision;
-    info->sweep_levels = ctx->nlev_f;
+    info->sweep_levels = ctx->nlev;
     info->device_bytes = ctx->device_bytes;
     return MFS_OK;
 }

@@ -59,6 +59,16 @@ __device__ __forceinline__ void block_minmax(double vmin, double vmax, double* o
     }
 }
 
+// Logical index of this workgroup.  Workgroups are dispatched round-robin
+// over the 8 XCDs, each with its own L2; mapping consecutive logical blocks
+// to one XCD gives each L2 a contiguous range of cells, whose neighbours
+// (and their faces and nodes) are mostly in the same range.  The grid is a
+// multiple of kXcd blocks.
+__device__ __forceinline__ int xcd_block() {
+    const int b = (int)blockIdx.x;
+    return (b % kXcd) * ((int)gridDim.x / kXcd) + b / kXcd;
+}
+
 __device__ __forceinline__ void load5(const double* __restrict__ p, double v[5]) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) v[j] = p[j];
@@ -91,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(int nC, const int* __restrict__
                                                  const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
                                                  const double* __restrict__ q, double* __restrict__ res,
                                                  double* __restrict__ part, FlowConst K) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int c = xcd_block() * kBlock + (int)threadIdx.x;
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (c < nC) {
         if (ACCUM) load5(res + (size_t)c * 5, r);
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_timestep(int nC, const int* __restri
                                                      const double* __restrict__ vol, double* __restrict__ dt,
                                                      double* __restrict__ diag, double* __restrict__ part, FlowConst K,
                                                      double cfl, double cfl_min) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int c = xcd_block() * kBlock + (int)threadIdx.x;
     double vmin = kBIGd, vmax = 0.0;
     if (c < nC) {
         double qc[5];
@@ -205,6 +215,97 @@ __global__ __launch_bounds__(kBlock) void k_timestep(int nC, const int* __restri
     block_minmax(vmin, vmax, part + (size_t)blockIdx.x * 8 + 5, part + (size_t)blockIdx.x * 8 + 6);
 }
 
+// Residual, local time step and LU-SGS diagonal in one pass over a cell's
+// faces, for the fused iteration: the arithmetic of k_flux<false> and
+// k_timestep<true>, which both read the state at the start of the
+// iteration.  The cell's own state is loaded once and only the neighbour's
+// is gathered per face.
+template <int MAXF>
+__global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                    const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                                    const double2* __restrict__ fdn, const double* __restrict__ q,
+                                                    const double* __restrict__ vol, double* __restrict__ res,
+                                                    double* __restrict__ dt, double* __restrict__ diag,
+                                                    double* __restrict__ part, FlowConst K, double cfl,
+                                                    double cfl_min) {
+    const int c = xcd_block() * kBlock + (int)threadIdx.x;
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double vmin = kBIGd, vmax = 0.0;
+    if (c < nC) {
+        double qc[5];
+        load5(q + (size_t)c * 5, qc);
+        int fs[MAXF];
+        const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
+        int2 cc[MAXF];
+        double4 g[MAXF];
+        double2 d[MAXF];
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            cc[s] = fcell[fs[s]];
+            g[s] = fgeo[fs[s]];
+            d[s] = fdn[fs[s]];
+        }
+        const double c2tmp = K.gam * (qc[4] + K.p_bar) / qc[0];
+        const double a = sqrt(c2tmp);
+        double dtc = kBIGd;
+        int det = 1;
+        double eig[MAXF];
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s) {
+            const bool first = (cc[s].x == c);  // c is the face's c1
+            double qo[5], ql[5], qr[5], F[5];
+            load5(q + (size_t)(first ? cc[s].y : cc[s].x) * 5, qo);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                ql[j] = first ? qc[j] : qo[j];
+                qr[j] = first ? qo[j] : qc[j];
+            }
+            roe_flux<double>(ql, qr, g[s].x, g[s].y, g[s].z, g[s].w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
+            const double dn = first ? d[s].x : d[s].y;
+            double vn = qc[1] * g[s].x + qc[2] * g[s].y + qc[3] * g[s].z;
+            vn = fabs(vn);
+            eig[s] = vn + a;
+            const double dp = fabs(qr[4] - ql[4]) / (qr[4] + ql[4] + K.p_bar + K.p_bar);
+            if (s < nf) {
+                if (first) {
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) r[j] -= F[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) r[j] += F[j];
+                }
+                dtc = smin(dtc, dn / eig[s]);
+                if (dp > 0.1) det = 0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) res[(size_t)c * 5 + j] = r[j];
+        const double p = qc[4];
+        double cfl_tmp;
+        if (p > K.p_break)
+            cfl_tmp = cfl;
+        else if (p < K.p_min)
+            cfl_tmp = cfl_min;
+        else
+            cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
+        if (!det) cfl_tmp = cfl_min;
+        dtc *= cfl_tmp;
+        dt[c] = dtc;
+        vmin = dtc;
+        vmax = dtc;
+        double dg = vol[c] / dtc;
+#pragma unroll
+        for (int s = 0; s < MAXF; ++s)
+            if (s < nf) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
+        diag[c] = dg;
+    }
+    double s2[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
+    block_sum<5>(s2, part + (size_t)blockIdx.x * 8);
+    block_minmax(vmin, vmax, part + (size_t)blockIdx.x * 8 + 5, part + (size_t)blockIdx.x * 8 + 6);
+}
+
 // Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
 // With `run`, only cells whose dt exceeds ratio_dtmax*dt_min are clamped
 // and recomputed (compute_timestep.cpp:929-945), and only when the
@@ -215,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(int nC, const int* __restrict__
                                                  const double* __restrict__ vol, double* __restrict__ dt,
                                                  double* __restrict__ diag, const double* __restrict__ run,
                                                  double ratio_dtmax, FlowConst K) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int c = xcd_block() * kBlock + (int)threadIdx.x;
     if (c >= nC) return;
     double dtc = dt[c];
     if (run) {
@@ -291,26 +392,39 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __res
 __global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
                                                  const double* __restrict__ nlw, const double* __restrict__ nden,
                                                  const double* __restrict__ q, double* __restrict__ qn) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_block() * kBlock + (int)threadIdx.x;
     if (p >= nN) return;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int e0 = nlo[p], e1 = nlo[p + 1];
-    for (int k = e0; k < e1; ++k) {
-        const int2 ab = nlab[k];
-        const double w = nlw[k];
-        double qa[5];
-        load5(q + (size_t)ab.x * 5, qa);
-        if (ab.y < 0) {
+    // entries in groups of U: the gathers of a group are issued together,
+    // then added in entry order
+    constexpr int U = 6;
+    for (int k = e0; k < e1; k += U) {
+        int2 ab[U];
+        double w[U], qa[U][5];
 #pragma unroll
-            for (int j = 0; j < 5; ++j) acc[j] += qa[j] * w;
-        } else {
-            double qb[5];
-            load5(q + (size_t)ab.y * 5, qb);
+        for (int i = 0; i < U; ++i) {
+            const int e = (k + i < e1) ? k + i : k;
+            ab[i] = nlab[e];
+            w[i] = nlw[e];
+        }
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                double fq = qa[j] + qb[j];
-                fq *= 0.5;
-                acc[j] += fq * w;
+        for (int i = 0; i < U; ++i) load5(q + (size_t)ab[i].x * 5, qa[i]);
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            if (k + i >= e1) break;
+            if (ab[i].y < 0) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) acc[j] += qa[i][j] * w[i];
+            } else {
+                double qb[5];
+                load5(q + (size_t)ab[i].y * 5, qb);
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    double fq = qa[i][j] + qb[j];
+                    fq *= 0.5;
+                    acc[j] += fq * w[i];
+                }
             }
         }
     }
@@ -345,17 +459,23 @@ __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __res
                                                  const double* __restrict__ q, const double* __restrict__ qn,
                                                  const double* __restrict__ vol, double* __restrict__ gx,
                                                  double* __restrict__ gy, double* __restrict__ gz) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int c = xcd_block() * kBlock + (int)threadIdx.x;
     if (c >= nC) return;
     double ax[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ay[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, az[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     int fs[MAXF];
     const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
     int2 cc[MAXF];
     double4 g[MAXF];
+    // tetrahedral meshes: every face is a triangle and its nodes sit at 3f
+    int tn[MAXF == 4 ? 12 : 1];
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
         cc[s] = fcell[fs[s]];
         g[s] = fgeo[fs[s]];
+        if constexpr (MAXF == 4) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tn[3 * s + i] = fn[3 * fs[s] + i];
+        }
     }
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
@@ -368,14 +488,29 @@ __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __res
             node_mean = (t == BC_INTERFACE || t == BC_SYMM);
         }
         if (node_mean) {
-            const int j0 = fno[f], j1 = fno[f + 1];
-            const double cnt = (double)(j1 - j0);
             double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int m = j0; m < j1; ++m) {
-                double v5[5];
-                load5(qn + (size_t)fn[m] * 5, v5);
+            double cnt;
+            if constexpr (MAXF == 4) {
+                double v0[5], v1[5], v2[5];
+                load5(qn + (size_t)tn[3 * s] * 5, v0);
+                load5(qn + (size_t)tn[3 * s + 1] * 5, v1);
+                load5(qn + (size_t)tn[3 * s + 2] * 5, v2);
 #pragma unroll
-                for (int j = 0; j < 5; ++j) sum[j] += v5[j];
+                for (int j = 0; j < 5; ++j) {
+                    sum[j] += v0[j];
+                    sum[j] += v1[j];
+                    sum[j] += v2[j];
+                }
+                cnt = 3.0;
+            } else {
+                const int j0 = fno[f], j1 = fno[f + 1];
+                cnt = (double)(j1 - j0);
+                for (int m = j0; m < j1; ++m) {
+                    double v5[5];
+                    load5(qn + (size_t)fn[m] * 5, v5);
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) sum[j] += v5[j];
+                }
             }
 #pragma unroll
             for (int j = 0; j < 5; ++j) qs[j] = sum[j] / cnt;

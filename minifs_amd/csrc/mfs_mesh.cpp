@@ -39,6 +39,129 @@ inline bool key_less(const FaceRec& l, const FaceRec& r) {
 
 }  // namespace
 
+namespace {
+
+// Hexahedra or prisms: every cell lists its faces with right-hand normals
+// pointing out of the cell, faces are matched by their sorted node sets, and
+// an interior face takes the node order of its lower-numbered cell (so its
+// normal points from c1 to c2).
+void build_poly_faces(RawMesh& m, const BoxBumpSpec& s) {
+    const int nx = s.nx, ny = s.ny, nz = s.nz;
+    const int px = nx + 1, py = ny + 1;
+    auto nid = [&](int i, int j, int k) { return i + px * (j + py * k); };
+    struct PF {
+        int key[4];  // sorted node ids, -1 padded
+        int cell;
+        int nn;
+        int n[4];    // outward order, smallest id first
+    };
+    std::vector<PF> recs;
+    recs.reserve((size_t)nx * ny * nz * (s.cells == CELLS_HEX ? 6 : 10));
+    auto add = [&](int cell, std::initializer_list<int> nodes) {
+        PF r;
+        r.cell = cell;
+        r.nn = (int)nodes.size();
+        int t[4], u = 0;
+        for (int v : nodes) t[u++] = v;
+        int rot = 0;
+        for (int a = 1; a < r.nn; ++a)
+            if (t[a] < t[rot]) rot = a;
+        for (int a = 0; a < r.nn; ++a) r.n[a] = t[(rot + a) % r.nn];
+        for (int a = 0; a < 4; ++a) r.key[a] = a < r.nn ? t[a] : -1;
+        std::sort(r.key, r.key + r.nn);
+        if (r.nn < 4) r.n[3] = -1;
+        recs.push_back(r);
+    };
+    int cell = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int v0 = nid(i, j, k), v1 = nid(i + 1, j, k), v2 = nid(i + 1, j + 1, k), v3 = nid(i, j + 1, k);
+                const int v4 = nid(i, j, k + 1), v5 = nid(i + 1, j, k + 1), v6 = nid(i + 1, j + 1, k + 1),
+                          v7 = nid(i, j + 1, k + 1);
+                if (s.cells == CELLS_HEX) {
+                    add(cell, {v0, v3, v2, v1});
+                    add(cell, {v4, v5, v6, v7});
+                    add(cell, {v0, v1, v5, v4});
+                    add(cell, {v3, v7, v6, v2});
+                    add(cell, {v0, v4, v7, v3});
+                    add(cell, {v1, v2, v6, v5});
+                    ++cell;
+                } else {
+                    // prism A: (0,1,2) / (4,5,6); prism B: (0,2,3) / (4,6,7), bottoms counter-clockwise
+                    const int pr[2][6] = {{v0, v1, v2, v4, v5, v6}, {v0, v2, v3, v4, v6, v7}};
+                    for (int h = 0; h < 2; ++h) {
+                        const int a = pr[h][0], b = pr[h][1], c = pr[h][2], a2 = pr[h][3], b2 = pr[h][4],
+                                  c2 = pr[h][5];
+                        add(cell, {a, c, b});
+                        add(cell, {a2, b2, c2});
+                        add(cell, {a, b, b2, a2});
+                        add(cell, {b, c, c2, b2});
+                        add(cell, {c, a, a2, c2});
+                        ++cell;
+                    }
+                }
+            }
+    m.nC = cell;
+    std::sort(recs.begin(), recs.end(), [](const PF& l, const PF& r) {
+        for (int a = 0; a < 4; ++a)
+            if (l.key[a] != r.key[a]) return l.key[a] < r.key[a];
+        return l.cell < r.cell;
+    });
+    struct Face {
+        int c1, c2, nn;
+        int n[4];
+    };
+    std::vector<Face> bf, inf;
+    auto same = [](const PF& l, const PF& r) {
+        for (int a = 0; a < 4; ++a)
+            if (l.key[a] != r.key[a]) return false;
+        return true;
+    };
+    for (size_t p = 0; p < recs.size();) {
+        size_t q = p + 1;
+        while (q < recs.size() && same(recs[q], recs[p])) ++q;
+        const PF& lo = recs[p];  // lower cell first (sort tie-break)
+        Face f;
+        f.c1 = lo.cell;
+        f.nn = lo.nn;
+        for (int a = 0; a < 4; ++a) f.n[a] = lo.n[a];
+        if (q - p == 1) {
+            bool bottom = true;
+            for (int a = 0; a < lo.nn; ++a) bottom = bottom && (lo.n[a] / (px * py) == 0);
+            f.c2 = bottom ? -1 : -2;
+            bf.push_back(f);
+        } else {
+            f.c2 = recs[p + 1].cell;
+            inf.push_back(f);
+        }
+        p = q;
+    }
+    std::sort(bf.begin(), bf.end(), [](const Face& l, const Face& r) {
+        if (l.c1 != r.c1) return l.c1 < r.c1;
+        if (l.n[0] != r.n[0]) return l.n[0] < r.n[0];
+        return l.n[1] < r.n[1];
+    });
+    std::sort(inf.begin(), inf.end(), [](const Face& l, const Face& r) {
+        if (l.c1 != r.c1) return l.c1 < r.c1;
+        return l.c2 < r.c2;
+    });
+    m.nB = (int)bf.size();
+    m.nF = (int)(bf.size() + inf.size());
+    m.nNPF.clear();
+    m.f2n.clear();
+    m.f2c.clear();
+    for (const std::vector<Face>* list : {&bf, &inf})
+        for (const Face& f : *list) {
+            m.nNPF.push_back(f.nn);
+            for (int a = 0; a < f.nn; ++a) m.f2n.push_back(f.n[a]);
+            m.f2c.push_back(f.c1);
+            m.f2c.push_back(f.c2);
+        }
+}
+
+}  // namespace
+
 RawMesh make_box_bump(const BoxBumpSpec& s) {
     RawMesh m;
     const int nx = s.nx, ny = s.ny, nz = s.nz;
@@ -70,6 +193,13 @@ RawMesh make_box_bump(const BoxBumpSpec& s) {
                 m.y[n] = yv;
                 m.z[n] = zv;
             }
+
+    if (s.cells != CELLS_TET) {
+        build_poly_faces(m, s);
+        m.bc_patch = {1, 2};
+        m.bc_type = {BC_WALL, BC_FAR_FIELD};
+        return m;
+    }
 
     // Kuhn split: tetrahedron t of a hex follows the axis permutation perm[t]
     // from corner (0,0,0) to corner (1,1,1).

@@ -29,8 +29,12 @@ struct RawMesh {
     std::vector<int> f2n_offsets() const;
 };
 
+// Cell kinds of the box generator.
+enum BoxCells : int { CELLS_TET = 0, CELLS_HEX = 1, CELLS_PRISM = 2 };
+
 struct BoxBumpSpec {
     int nx = 20, ny = 20, nz = 20;
+    int cells = CELLS_TET;       // tetrahedra (6 per hex), hexahedra, or triangular prisms (2 per hex)
     double jitter = 0.10;        // interior node jitter, fraction of the local spacing
     double bump_height = 0.05;   // Gaussian bump on the z=0 wall
     double bump_sigma = 0.10;
@@ -42,6 +46,10 @@ struct BoxBumpSpec {
 // displaced by the bump, blended linearly to zero at z=1; other five sides
 // are patch 2 (FAR_FIELD).  Boundary faces first, faces oriented so the
 // right-hand normal points from c1 to c2 (outward on the boundary).
+// With cells = CELLS_HEX each hex is one cell (6 quadrilateral faces); with
+// CELLS_PRISM each hex is split along its (0,0)-(1,1) xy diagonal into two
+// triangular prisms (2 triangles + 3 quadrilaterals).  Same nodes, bump,
+// patches and face ordering as the tetrahedral mesh.
 RawMesh make_box_bump(const BoxBumpSpec& spec);
 
 // FlowStar binary writer/reader (format of load_grid.cpp:129-273, with the

@@ -10,6 +10,9 @@
 
 namespace mfs {
 
+// Cells per storage tile of the device order (a power of two is not needed).
+constexpr int kTileCells = 4096;
+
 namespace {
 constexpr double kTINY_ = 1.e-40;
 }
@@ -76,8 +79,7 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
     // of cell c reads the final DQ of every face neighbour with a smaller id
     // and the backward sweep that of every neighbour with a larger id.
     L.flevel.assign(nC, 0);
-    L.blevel.assign(nC, 0);
-    int mf = 0, mb = 0;
+    int mf = 0;
     for (int c = 0; c < nC; ++c) {
         int l = 0;
         for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
@@ -89,29 +91,24 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         L.flevel[c] = l;
         mf = std::max(mf, l);
     }
-    for (int c = nC - 1; c >= 0; --c) {
-        int l = 0;
-        for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
-            const int f = mcf[k];
-            if (f < nB) continue;
-            const int nb = other(c, f);
-            if (nb > c) l = std::max(l, L.blevel[nb] + 1);
-        }
-        L.blevel[c] = l;
-        mb = std::max(mb, l);
-    }
-    L.nlev_f = mf + 1;
-    L.nlev_b = mb + 1;
+    L.nlev = mf + 1;
 
-    // ---- device cell order: forward level, then mesh id
+    // ---- device cell order: tile (kTileCells consecutive mesh ids, a
+    // compact region in TDTree numbering), then forward level, then mesh id.
+    // The cells of one level inside a tile are contiguous (sweep chunks),
+    // and a cell's face neighbours are mostly in its own tile (gathers of
+    // the face kernels stay in a small window of memory).
     std::vector<int> dcell(nC);
     {
-        std::vector<int> cnt(L.nlev_f + 1, 0);
-        for (int c = 0; c < nC; ++c) cnt[L.flevel[c] + 1]++;
-        for (int l = 0; l < L.nlev_f; ++l) cnt[l + 1] += cnt[l];
+        const int ntile = (nC + kTileCells - 1) / kTileCells;
+        const size_t nkey = (size_t)ntile * L.nlev;
+        std::vector<int> cnt(nkey + 1, 0);
+        auto key = [&](int c) { return (size_t)(c / kTileCells) * L.nlev + L.flevel[c]; };
+        for (int c = 0; c < nC; ++c) cnt[key(c) + 1]++;
+        for (size_t k = 0; k < nkey; ++k) cnt[k + 1] += cnt[k];
         L.cell_of_dev.resize(nC);
         for (int c = 0; c < nC; ++c) {
-            const int d = cnt[L.flevel[c]]++;
+            const int d = cnt[key(c)]++;
             dcell[c] = d;
             L.cell_of_dev[d] = c;
         }
@@ -363,46 +360,71 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         }
     }
 
-    // ---- padded level schedules (device ids) and the latest dependency of each chunk
-    auto build_sched = [&](const std::vector<int>& lev, int nlev, bool fwd, std::vector<int>& sched,
-                           std::vector<int>& rep) {
-        std::vector<int> cnt(nlev, 0);
-        for (int c = 0; c < nC; ++c) cnt[lev[c]]++;
-        std::vector<int> start(nlev + 1, 0);
-        for (int l = 0; l < nlev; ++l) start[l + 1] = start[l] + (cnt[l] + 63) / 64 * 64;
-        sched.assign(start[nlev], -1);
-        std::vector<int> pos(nC);
-        std::vector<int> fillp(start.begin(), start.end() - 1);
-        if (fwd) {
-            for (int c = 0; c < nC; ++c) pos[c] = fillp[lev[c]]++;
-        } else {
-            for (int c = nC - 1; c >= 0; --c) pos[c] = fillp[lev[c]]++;
+    // ---- sweep chunks.  A chunk is up to `cpc` consecutive device cells of
+    // one tile and one forward level, so its cells never depend on one
+    // another.  The forward sweep runs the chunks by level (then tile); the
+    // backward sweep runs them in the reverse order, which is valid for it
+    // because an upper neighbour always has a higher forward level.  `chunk`
+    // lists the chunks in forward run order; rep_f[u] is the dependency of
+    // forward chunk u in the latest chunk of the run order, rep_b[u] the same
+    // for the chunk the backward sweep runs u-th (forward chunk nchunk-1-u).
+    L.lpc = L.max_faces <= 4 ? 4 : 8;
+    L.cpc = 64 / L.lpc;
+    {
+        struct Ch {
+            int level, start, count;
+        };
+        std::vector<Ch> chs;
+        for (int d = 0; d < nC;) {
+            const int c0 = L.cell_of_dev[d];
+            const int l = L.flevel[c0], tile = c0 / kTileCells;
+            int e = d;
+            while (e < nC && e - d < L.cpc && L.flevel[L.cell_of_dev[e]] == l &&
+                   L.cell_of_dev[e] / kTileCells == tile)
+                ++e;
+            chs.push_back({l, d, e - d});
+            d = e;
         }
-        for (int c = 0; c < nC; ++c) sched[pos[c]] = dcell[c];
-        const int nchunk = start[nlev] / 64;
-        rep.assign(nchunk, -1);
-        std::vector<int> repPos(nchunk, -1);
-        for (int c = 0; c < nC; ++c) {
-            const int ch = pos[c] / 64;
+        std::stable_sort(chs.begin(), chs.end(), [](const Ch& a, const Ch& b) { return a.level < b.level; });
+        const int nchunk = (int)chs.size();
+        std::vector<int> run_of(nC);
+        L.chunk.resize(2 * (size_t)nchunk);
+        for (int u = 0; u < nchunk; ++u) {
+            L.chunk[2 * u] = chs[u].start;
+            L.chunk[2 * u + 1] = chs[u].count;
+            for (int x = chs[u].start; x < chs[u].start + chs[u].count; ++x) run_of[x] = u;
+        }
+        L.rep_f.assign(nchunk, -1);
+        L.rep_b.assign(nchunk, -1);
+        std::vector<int> bestF(nchunk, -1), bestB(nchunk, nchunk);
+        for (int d = 0; d < nC; ++d) {
+            const int c = L.cell_of_dev[d];
+            const int t = run_of[d];
             for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
                 const int f = mcf[k];
                 if (f < nB) continue;
                 const int nb = other(c, f);
-                if (fwd ? (nb < c) : (nb > c)) {
-                    if (pos[nb] / 64 >= ch) return std::string("sweep schedule is not topological");
-                    if (pos[nb] > repPos[ch]) {
-                        repPos[ch] = pos[nb];
-                        rep[ch] = dcell[nb];
+                const int tn = run_of[dcell[nb]];
+                if (nb < c) {
+                    if (tn >= t) return std::string("forward sweep chunks are not topological");
+                    if (tn > bestF[t]) {
+                        bestF[t] = tn;
+                        L.rep_f[t] = dcell[nb];
+                    }
+                } else {
+                    if (tn <= t) return std::string("backward sweep chunks are not topological");
+                    if (tn < bestB[t]) {
+                        bestB[t] = tn;
+                        L.rep_b[nchunk - 1 - t] = dcell[nb];
                     }
                 }
             }
         }
-        return std::string();
-    };
-    std::string e = build_sched(L.flevel, L.nlev_f, true, L.sched_f, L.rep_f);
-    if (!e.empty()) return e;
-    e = build_sched(L.blevel, L.nlev_b, false, L.sched_b, L.rep_b);
-    if (!e.empty()) return e;
+        // face slots per device cell (device face ids, ascending mesh face id, -1 = none)
+        L.cfs.assign((size_t)nC * L.lpc, -1);
+        for (int d = 0; d < nC; ++d)
+            for (int k = L.cfo[d]; k < L.cfo[d + 1]; ++k) L.cfs[(size_t)d * L.lpc + (k - L.cfo[d])] = L.cf[k];
+    }
 
     // ---- state, device order (ghosts unchanged)
     L.q.resize(5 * (size_t)n);

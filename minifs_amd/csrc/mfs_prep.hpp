@@ -9,9 +9,8 @@
 //
 // Storage order on the device is free as long as that order of operations
 // is kept.  Cells are stored in the order of the forward LU-SGS schedule
-// (DAG level, then mesh id), so the 64 cells of a sweep chunk are
-// contiguous; faces and nodes are stored in first-touch order of that cell
-// traversal.  Everything that depends on mesh numbering (per-cell face
+// (DAG level, then mesh id), so the cells of a sweep chunk are contiguous;
+// faces and nodes are stored in first-touch order of that cell traversal.  Everything that depends on mesh numbering (per-cell face
 // order, the sweep direction test "neighbour id < cell id", the per-node
 // summation order) is evaluated here on mesh ids and stored as lists or
 // masks.
@@ -56,13 +55,15 @@ struct HostLayout {
     std::vector<double> nden;                  // nN: WeightNode + TINY
     std::vector<int> sym_node;                 // device nodes with nodesymm == 1
     std::vector<double> sym_n;                 // 3 per symmetry node (xfn_n_symm, ...)
-    // LU-SGS sweep schedule: device cell ids by DAG level (mesh-id levels),
-    // each level padded with -1 to a multiple of 64 so that no 64-cell
-    // chunk spans two levels.
-    std::vector<int> flevel, blevel;           // nC, by mesh id
-    std::vector<int> sched_f, sched_b;         // padded schedules
-    std::vector<int> rep_f, rep_b;             // per chunk: dependency published last in schedule order (-1: none)
-    int nlev_f = 0, nlev_b = 0;
+    // LU-SGS sweeps: forward DAG level of each mesh cell, the number of
+    // levels, and the chunks (device cell ranges of one level, `cpc` cells,
+    // `lpc` lanes per cell) with their latest dependencies
+    std::vector<int> flevel;                   // nC, by mesh id
+    int nlev = 0;
+    int lpc = 4, cpc = 16;
+    std::vector<int> chunk;                    // 2 per chunk: first device cell, cell count
+    std::vector<int> rep_f, rep_b;             // per chunk in run order: device cell published last (-1: none)
+    std::vector<int> cfs;                      // lpc slots per device cell: device face id or -1
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
 };

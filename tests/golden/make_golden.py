@@ -10,6 +10,10 @@ Fixtures (all little-endian raw arrays, plus manifest.json):
                 Mach 0.5, AoA 0, CFL 10: the reference's 100-iteration
                 history and SHA-256 digests of its state after 1 and 100
                 iterations (the arrays themselves are too large to commit).
+  hex4_*, prism4_* : the same box with 4x4x4 hexahedra (6 quadrilateral
+                faces per cell) and with 2x4x4x4 triangular prisms; mesh
+                files, the reference's 10-iteration history at Mach 0.8,
+                AoA 1.25, CFL 10 and digests of its state after 10 iterations.
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
@@ -77,6 +81,18 @@ def main():
         man["cases"]["box20"] = {"n": 20, "mach": 0.5, "alpha": 0.0, "cfl": 10.0, "iters": 100,
                                  "mesh_sha256": sha(mesh), "state_sha256": dig,
                                  "final_line": open(os.path.join(out, "summary.txt")).read().splitlines()[-1]}
+        # ---- hex4, prism4: non-tetrahedral cells
+        for cells in ("hex", "prism"):
+            name = f"{cells}4"
+            g = mfs.Grid.box_bump(4, cells=cells).decouple()
+            mesh, bc = os.path.join(HERE, f"{name}.mfl"), os.path.join(HERE, f"{name}.bc")
+            g.save(mesh, bc)
+            out = os.path.join(td, name)
+            run_ref(mesh, bc, out, 0.8, 1.25, 10.0, 10, (10,))
+            shutil.copy(os.path.join(out, "history.f64"), os.path.join(HERE, f"{name}_history.f64"))
+            dig = {f"it10_{nm}": sha(os.path.join(out, f"it10_{nm}.f64")) for nm in FIELDS}
+            man["cases"][name] = {"n": 4, "cells": cells, "mach": 0.8, "alpha": 1.25, "cfl": 10.0, "iters": 10,
+                                  "mesh_sha256": sha(mesh), "state_sha256": dig}
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(man, f, indent=1, sort_keys=True)
     print("golden fixtures written")

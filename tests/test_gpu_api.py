@@ -1,0 +1,100 @@
+"""GPU tests of the C ABI beyond the reference build: the HIP path against
+the oracle (C restatement) on a larger mesh and another flow condition,
+state upload/download, error reporting, and independent contexts."""
+import os
+
+import numpy as np
+import pytest
+
+import minifs_amd as mfs
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p", "res", "dt", "dq", "dqdx", "dqdy", "dqdz")
+
+
+def _mesh(tmp_path, n, seed=20260101):
+    g = mfs.Grid.box_bump(n, seed=seed).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    return g, mesh, bc
+
+
+@pytest.mark.parametrize("n,mach,alpha,cfl,iters", [(40, 0.8, 1.25, 10.0, 4), (12, 0.3, -3.0, 5.0, 25)])
+def test_step_bitwise_vs_oracle(tmp_path, n, mach, alpha, cfl, iters):
+    print(f"[api] oracle parity N={n} M={mach} a={alpha} cfl={cfl} iters={iters}", flush=True)
+    g, mesh, bc = _mesh(tmp_path, n)
+    o = Oracle(mesh, bc, mach, alpha, cfl)
+    hist_o, _ = o.run(iters, min(16, os.cpu_count() or 1))
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
+    g.preprocess(p)
+    s = mfs.Solver(g, p)
+    s.step(iters)
+    for nm in FIELDS:
+        a, b = s.field(nm), o.get(nm)
+        bad = np.flatnonzero(a != b)
+        assert bad.size == 0, f"{nm}: {bad.size} entries differ, first {bad[:4]}"
+    h = s.history()
+    np.testing.assert_array_equal(h[:, 5:], hist_o[:, 5:])
+    np.testing.assert_allclose(h[:, :5], hist_o[:, :5], rtol=1e-12, atol=0)
+    s.close()
+    o.close()
+
+
+def test_upload_download_round_trip_and_restart(tmp_path):
+    """A state uploaded into a fresh context continues the solve exactly
+    where the first context was (the solve has no hidden state besides the
+    running dt extrema and the iteration counter)."""
+    p = mfs.make_params(mach=0.6, alpha=2.0, cfl=10.0)
+    g = mfs.Grid.box_bump(10).decouple().preprocess(p)
+    a = mfs.Solver(g, p)
+    a.step(6)
+    snap = {nm: a.field(nm) for nm in ("rho", "u", "v", "w", "p")}
+    a.step(1)
+    b = mfs.Solver(g, p)
+    for nm, v in snap.items():
+        b.upload(nm, v)
+        np.testing.assert_array_equal(b.field(nm), v)
+    b.step(1)
+    for nm in ("rho", "u", "v", "w", "p", "res", "dq"):
+        np.testing.assert_array_equal(a.field(nm), b.field(nm), err_msg=nm)
+    a.close()
+    b.close()
+
+
+def test_errors_are_reported():
+    p = mfs.make_params(mach=0.5, alpha=0.0, cfl=10.0)
+    g = mfs.Grid.box_bump(3).decouple().preprocess(p)
+    s = mfs.Solver(g, p)
+    n = g.info["nTCell"] + g.info["nBFace"]
+    L = mfs.lib()
+    buf = np.zeros(n)
+    dp = buf.ctypes.data_as(mfs._lib._DP)
+    assert L.mfs_download(s._h, 99, dp, n) < 0
+    assert L.mfs_download(s._h, mfs._lib.F_RHO, dp, n - 1) < 0
+    grad = np.zeros(5 * n)
+    assert L.mfs_upload(s._h, mfs._lib.F_DQDX, grad.ctypes.data_as(mfs._lib._DP), 5 * n) < 0  # read-only
+    assert L.mfs_step(s._h, -1) < 0
+    assert "negative" in L.mfs_last_error().decode()
+    # the context stays usable after argument errors
+    s.step(2)
+    assert s.history().shape == (2, 7)
+    sc = s.scalars()
+    assert sc["iter_done"] == 2
+    assert sc["dt_max"] == s.history()[-1, 5] and sc["dt_min"] == s.history()[-1, 6]
+    s.close()
+
+
+def test_contexts_are_independent(tmp_path):
+    p = mfs.make_params(mach=0.7, alpha=1.0, cfl=10.0)
+    g = mfs.Grid.box_bump(8).decouple().preprocess(p)
+    a, b = mfs.Solver(g, p), mfs.Solver(g, p)
+    a.step(3)
+    b.step(1)
+    b.step(2)
+    for nm in FIELDS:
+        np.testing.assert_array_equal(a.field(nm), b.field(nm), err_msg=nm)
+    np.testing.assert_array_equal(a.history(), b.history())
+    a.close()
+    b.close()

@@ -60,6 +60,29 @@ def test_generator_counts_and_orientation(n):
     assert (g.array("vol") > 0).all()
 
 
+@pytest.mark.parametrize("cells,per_hex,faces_per_cell", [("hex", 1, 6), ("prism", 2, 5)])
+def test_polyhedral_generator(cells, per_hex, faces_per_cell):
+    nx, ny, nz = 3, 4, 2
+    g = mfs.Grid.box_bump(nx, ny, nz, cells=cells)
+    info = g.info
+    assert info["nTCell"] == per_hex * nx * ny * nz
+    assert info["nTNode"] == (nx + 1) * (ny + 1) * (nz + 1)
+    nnpf = g.array("nNPF")
+    f2c = g.array("f2c").reshape(-1, 2)
+    nB = info["nBFace"]
+    # every cell has faces_per_cell faces; quadrilaterals on the box sides
+    cnt = np.bincount(np.concatenate([f2c[:, 0], f2c[nB:, 1]]), minlength=info["nTCell"])
+    assert (cnt == faces_per_cell).all()
+    assert set(np.unique(nnpf)) == ({4} if cells == "hex" else {3, 4})
+    assert (f2c[:nB, 1] < 0).all() and (f2c[nB:, 1] >= 0).all()
+    assert (f2c[nB:, 0] < f2c[nB:, 1]).all()
+    g.preprocess(mfs.make_params(0.5, 0.0, 10.0))
+    vol = g.array("vol")[:info["nTCell"]]
+    assert (vol > 0).all()
+    # closed surface: the cell volumes fill the box minus the bump
+    assert abs(vol.sum() - 1.0) < 0.01
+
+
 def test_generator_is_deterministic_and_seeded():
     a = mfs.Grid.box_bump(5, seed=3).array("x")
     b = mfs.Grid.box_bump(5, seed=3).array("x")

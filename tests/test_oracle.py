@@ -71,6 +71,21 @@ def test_config1_history_and_digests(tmp_path):
     np.testing.assert_allclose(h[-1], [float(x) for x in last[1:]], rtol=5e-6)
 
 
+@pytest.mark.parametrize("case", ["hex4", "prism4"])
+@pytest.mark.parametrize("threads", [0, 4])
+def test_polyhedral_meshes_history_and_digests(case, threads):
+    """Hexahedra (quadrilateral faces, 6 per cell) and prisms (mixed
+    triangle/quadrilateral faces, 5 per cell): the oracle reproduces the
+    reference's 10-iteration history and state bit for bit."""
+    c = MAN["cases"][case]
+    o = Oracle(os.path.join(GOLD, f"{case}.mfl"), os.path.join(GOLD, f"{case}.bc"), c["mach"], c["alpha"], c["cfl"])
+    h, _ = o.run(c["iters"], threads)
+    np.testing.assert_array_equal(h, gold(f"{case}_history.f64").reshape(-1, 7))
+    for nm in FIELDS:
+        d = hashlib.sha256(o.get(nm).tobytes()).hexdigest()
+        assert d == c["state_sha256"][f"it10_{nm}"], nm
+
+
 def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver):
     """Where the reference build exists: a mesh not in the fixtures, other flow."""
     import minifs_amd as mfs
