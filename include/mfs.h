@@ -207,6 +207,28 @@ int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
 int mfs_ctx_get_timing(mfs_ctx* ctx, const char* name, double* ms, int* launches);
 int mfs_ctx_reset_timing(mfs_ctx* ctx);
 
+/* ------------------------------------------------------------------ */
+/* Batched multi-solve.  No counterpart in the reference, which runs one
+ * solve per process (src/main.cpp:198-291).  A batch steps several
+ * independent solves, each with its own mesh, flow condition and device
+ * arrays, with one launch per kernel for all of them (the sweeps walk a
+ * list of (solve, tile) items).  Each solve's arithmetic is the
+ * single-solve path's, so its results are the same alone and in a batch.
+ * All solves of a batch share one precision and one cell family
+ * (tetrahedral meshes, or meshes with up to 6 faces per cell).          */
+typedef struct mfs_batch mfs_batch;
+int mfs_batch_create(int precision, mfs_batch** out);
+int mfs_batch_destroy(mfs_batch* b);
+/* Builds the solve's device layout and uploads it; *slot names it until
+ * removed.  mfs_batch_ctx(b, slot) gives its context for mfs_get_history,
+ * mfs_download, mfs_get_scalars (do not destroy it directly). */
+int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_params* p, int* slot);
+int mfs_batch_remove(mfs_batch* b, int slot);
+int mfs_batch_step(mfs_batch* b, int n); /* n iterations of every solve of the batch */
+int mfs_batch_sync(mfs_batch* b);
+mfs_ctx* mfs_batch_ctx(mfs_batch* b, int slot);
+int mfs_batch_count(mfs_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

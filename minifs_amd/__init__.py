@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib
 from ._lib import FIELD_IDS, MFS_FP64, MFS_MIXED, MfsError, check, lib
 
-__all__ = ["Grid", "Solver", "make_params", "MFS_FP64", "MFS_MIXED", "MfsError", "lib"]
+__all__ = ["Grid", "Solver", "Batch", "make_params", "MFS_FP64", "MFS_MIXED", "MfsError", "lib"]
 
 BC_WALL, BC_SYMM, BC_FAR_FIELD, BC_FAR_FIELD_MOD, BC_INTERFACE = 3, 4, 6, 61, 10
 
@@ -204,6 +204,69 @@ class Solver:
     def close(self):
         if self._h:
             lib().mfs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _BatchMember(Solver):
+    """A solve owned by a Batch: the Solver queries on its context; the batch
+    frees it."""
+
+    def __init__(self, batch: "Batch", slot: int, grid: Grid, params: _lib.MfsParams):
+        self._h = C.c_void_p(lib().mfs_batch_ctx(batch._h, slot))
+        self.params = params
+        self.grid = grid
+        self.slot = slot
+        self._batch = batch
+
+    def close(self):
+        self._h = None  # owned by the batch
+
+    def solve(self, *a, **k):
+        raise RuntimeError("step a batch member through its Batch")
+
+
+class Batch:
+    """Several independent solves stepped together (mfs_batch_*)."""
+
+    def __init__(self, precision: int = MFS_FP64):
+        self._h = C.c_void_p()
+        check(lib().mfs_batch_create(precision, C.byref(self._h)), "mfs_batch_create")
+        self.members: dict[int, _BatchMember] = {}
+
+    def add(self, grid: Grid, params: _lib.MfsParams) -> _BatchMember:
+        view = grid.view()
+        slot = C.c_int()
+        check(lib().mfs_batch_add(self._h, C.byref(view), C.byref(params), C.byref(slot)), "mfs_batch_add")
+        m = _BatchMember(self, slot.value, grid, params)
+        self.members[slot.value] = m
+        return m
+
+    def remove(self, member: _BatchMember):
+        check(lib().mfs_batch_remove(self._h, member.slot), "mfs_batch_remove")
+        self.members.pop(member.slot, None)
+        member._h = None
+
+    def step(self, n: int = 1):
+        check(lib().mfs_batch_step(self._h, n), "mfs_batch_step")
+
+    def sync(self):
+        check(lib().mfs_batch_sync(self._h), "mfs_batch_sync")
+
+    def __len__(self):
+        return lib().mfs_batch_count(self._h)
+
+    def close(self):
+        if self._h:
+            for m in self.members.values():
+                m._h = None
+            self.members.clear()
+            lib().mfs_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -98,6 +98,14 @@ _SIGS = {
     "mfs_ctx_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "mfs_ctx_get_timing": (C.c_int, [C.c_void_p, C.c_char_p, _DP, _IP]),
     "mfs_ctx_reset_timing": (C.c_int, [C.c_void_p]),
+    "mfs_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mfs_batch_destroy": (C.c_int, [C.c_void_p]),
+    "mfs_batch_add": (C.c_int, [C.c_void_p, C.POINTER(MfsMeshView), C.POINTER(MfsParams), C.POINTER(C.c_int)]),
+    "mfs_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
+    "mfs_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "mfs_batch_sync": (C.c_int, [C.c_void_p]),
+    "mfs_batch_ctx": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "mfs_batch_count": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,105 @@
+// Batched multi-solve kernels, included by mfs_device.hip inside namespace
+// mfs::dev after the per-solve kernels.
+//
+// A batch steps several independent solves (each its own mesh, flow
+// condition and device arrays) with one launch per kernel for all of them.
+// Each launch reads a table with one BatchSolve entry per solve; the cell
+// and node kernels map their workgroups to solves through a block map (each
+// solve owns a whole number of XCD groups of 8 workgroups), and the sweeps
+// walk a list of (solve, tile) items.  The per-cell arithmetic is the
+// single-solve kernels' own (the same device functions), so a solve gives
+// the same bits alone and in a batch.
+
+template <typename R>
+struct BatchSolve {
+    int nC, nN, n;                // cells, nodes, cells + ghosts
+    int blk0_cell, nblk_cell;     // global block range of the cell kernels
+    int blk0_node, nblk_node;     // and of the node kernel
+    const int *cfo, *cf;
+    const int2* fcell;
+    const double4* fgeo;
+    const double2* fdn;
+    const double* vol;
+    const double* q;              // state at the start of the iteration
+    const double* qnew;           // state written by the backward sweep
+    double *res, *dt, *diag, *part, *run, *hist;
+    int hist_slot;
+    double cfl, cfl_min, ratio_dtmax;
+    float *q32, *diag32;          // mixed precision only
+    FlowConst K;
+    // gradient
+    const int* nlo;
+    const int2* nlab;
+    const R* nlw;
+    const double* nden;
+    R* qn;
+    const int *fno, *fn, *btype;
+    const typename Real<R>::vec4* fgeo_r;
+    R *gx, *gy, *gz;
+    // sweeps
+    SweepArgs<R> sw;
+};
+
+template <int MAXF, bool MIXED, typename R>
+__global__ __launch_bounds__(kBlock) void k_b_flux_ts(const BatchSolve<R>* __restrict__ T,
+                                                      const int* __restrict__ cell_solve) {
+    const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
+    const int lb = (int)blockIdx.x - S.blk0_cell;
+    const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;  // XCD-aware inside the solve
+    flux_ts_body<MAXF, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn, S.q, S.vol,
+                              S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min, S.q32, S.diag32);
+}
+
+// residual norms, dt extrema and the history record of every solve (one
+// workgroup per solve)
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_b_reduce(const BatchSolve<R>* __restrict__ T) {
+    const BatchSolve<R>& S = T[blockIdx.x];
+    reduce_body(S.nblk_cell, S.part, S.run, S.hist, S.hist_slot, 1 | 2 | 4, S.ratio_dtmax);
+}
+
+template <int MAXF, typename R>
+__global__ __launch_bounds__(kBlock) void k_b_clamp(const BatchSolve<R>* __restrict__ T,
+                                                    const int* __restrict__ cell_solve) {
+    const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
+    const int lb = (int)blockIdx.x - S.blk0_cell;
+    const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
+    diag_body<MAXF>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fgeo, S.q, S.vol, S.dt, S.diag, S.run,
+                    S.ratio_dtmax, S.K, S.q32 ? S.diag32 : nullptr);
+}
+
+template <bool FWD, int LPC, typename R>
+__global__ __launch_bounds__(kSweepBlock) void k_b_sweep(const BatchSolve<R>* __restrict__ T,
+                                                        const int2* __restrict__ items, int nitems) {
+    const int W = (int)blockDim.x / 64;
+    const int w = (int)threadIdx.x / 64;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 sk = items[it];  // (solve, tile), tiles in sweep order within a solve
+        const SweepArgs<R>& A = T[sk.x].sw;
+        if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
+        const int c0 = A.tile_chunk[sk.y], c1 = A.tile_chunk[sk.y + 1];
+        for (int j = w; j < c1 - c0; j += W) {
+            const int t = FWD ? c0 + j : c1 - 1 - j;
+            if (!sweep_chunk<FWD, LPC, R>(A, T[sk.x].K, t)) break;
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_b_node(const BatchSolve<R>* __restrict__ T,
+                                                   const int* __restrict__ node_solve) {
+    const BatchSolve<R>& S = T[node_solve[blockIdx.x]];
+    const int lb = (int)blockIdx.x - S.blk0_node;
+    const int lg = (lb % kXcd) * (S.nblk_node / kXcd) + lb / kXcd;
+    node_body<R>(lg * kBlock + (int)threadIdx.x, S.nN, S.nlo, S.nlab, S.nlw, S.nden, S.qnew, S.qn);
+}
+
+template <int MAXF, typename R>
+__global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restrict__ T,
+                                                   const int* __restrict__ cell_solve) {
+    const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
+    const int lb = (int)blockIdx.x - S.blk0_cell;
+    const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
+    grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
+                       S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
+}

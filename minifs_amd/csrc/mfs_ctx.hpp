@@ -26,6 +26,7 @@ struct KernelTimer {
 struct mfs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;  // false for a solve of a batch (the batch's stream)
     int nC = 0, nF = 0, nB = 0, nN = 0;
     int precision = MFS_FP64;
     int maxf = 4;  // face slots per cell used by the kernels (4 or 6)
@@ -45,6 +46,16 @@ struct mfs_ctx {
     double *dq = nullptr;   // final DQ, 5 per cell incl. ghosts (0)
     double *gx = nullptr, *gy = nullptr, *gz = nullptr;  // reference layout: 5 blocks of nC+nB
     double* qn = nullptr;   // node values, 5 per device node
+    // mixed precision (MFS_MIXED): FP32 implicit operator and gradients; the
+    // FP64 arrays above keep the residual, time step and state
+    float* q32 = nullptr;     // state snapshot of the iteration, 5 per cell incl. ghosts
+    float* diag32 = nullptr;
+    float* dqf32 = nullptr;   // forward-sweep DQ
+    float* dq32 = nullptr;    // final DQ incl. ghosts (0)
+    float *gx32 = nullptr, *gy32 = nullptr, *gz32 = nullptr;
+    float* qn32 = nullptr;
+    float* nlw32 = nullptr;   // node weights
+    float4* fgeo32 = nullptr; // unit normal + area
     // faces (device order)
     int2* fcell = nullptr;
     double4* fgeo = nullptr;
@@ -63,7 +74,9 @@ struct mfs_ctx {
     std::vector<int> node_of_dev;  // host: device node -> mesh node
     std::vector<int> cell_of_dev;  // host: device cell -> mesh cell
     // sweeps
-    int2* chunk = nullptr;  // per chunk: first device cell, cell count
+    int2* chunk = nullptr;  // per chunk (device order): first device cell, cell count
+    int* tile_chunk = nullptr;  // ntiles+1: first chunk of each tile
+    int ntiles = 0;
     int *rep_f = nullptr, *rep_b = nullptr;
     int* cfs = nullptr;     // lpc face slots per device cell
     int* depmask = nullptr;

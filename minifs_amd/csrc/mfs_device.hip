@@ -28,10 +28,12 @@ namespace dev {
 constexpr int kBlock = 256;
 constexpr int kXcd = 8;  // XCDs of an MI355X; cell-kernel grids are a multiple of this
 constexpr int kSweepWave = 64;
+constexpr int kSweepBlock = 512;  // waves of a sweep workgroup x 64: one tile per workgroup
 constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wave
 
 #include "mfs_kernels.hpp"
 #include "mfs_sweep.hpp"
+#include "mfs_batch.hpp"
 
 }  // namespace dev
 
@@ -180,13 +182,26 @@ bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
 template <int MAXF>
 bool launch_flux_ts_t(mfs_ctx* ctx, double cfl, double cfl_min) {
     Timed t(ctx, "flux_ts");
-    hipLaunchKernelGGL((dev::k_flux_ts<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->cfo,
-                       ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res, ctx->dt,
-                       ctx->diag, ctx->part, ctx->K, cfl, cfl_min);
+    if (ctx->precision == MFS_MIXED)
+        hipLaunchKernelGGL((dev::k_flux_ts<MAXF, true>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res,
+                           ctx->dt, ctx->diag, ctx->part, ctx->K, cfl, cfl_min, ctx->q32, ctx->diag32);
+    else
+        hipLaunchKernelGGL((dev::k_flux_ts<MAXF, false>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res,
+                           ctx->dt, ctx->diag, ctx->part, ctx->K, cfl, cfl_min, nullptr, nullptr);
     return check_launch(ctx, "k_flux_ts");
 }
 bool launch_flux_ts(mfs_ctx* ctx, double cfl, double cfl_min) {
     return ctx->maxf == 4 ? launch_flux_ts_t<4>(ctx, cfl, cfl_min) : launch_flux_ts_t<6>(ctx, cfl, cfl_min);
+}
+
+// Mixed precision, per-entry-point path: the FP32 operator inputs from the
+// current state and Diag (mfs_step writes them in the fused kernel).
+bool launch_mixed_inputs(mfs_ctx* ctx) {
+    hipLaunchKernelGGL(dev::k_mixed_inputs, dim3(nblk(ctx->nC)), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                       ctx->q[ctx->cur], ctx->diag, ctx->q32, ctx->diag32);
+    return check_launch(ctx, "k_mixed_inputs");
 }
 
 bool launch_reduce(mfs_ctx* ctx, int mode, int slot) {
@@ -201,39 +216,51 @@ bool launch_diag_t(mfs_ctx* ctx, bool clamp_only) {
     Timed t(ctx, clamp_only ? "clamp" : "diag");
     hipLaunchKernelGGL((dev::k_diag<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->cfo,
                        ctx->cf, ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt, ctx->diag,
-                       clamp_only ? ctx->run : nullptr, ctx->P.ratio_dtmax, ctx->K);
+                       clamp_only ? ctx->run : nullptr, ctx->P.ratio_dtmax, ctx->K, ctx->diag32);
     return check_launch(ctx, "k_diag");
 }
 bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
     return ctx->maxf == 4 ? launch_diag_t<4>(ctx, clamp_only) : launch_diag_t<6>(ctx, clamp_only);
 }
 
-template <int LPC>
+template <int LPC, typename R>
 bool launch_sweeps_t(mfs_ctx* ctx) {
-    dev::SweepArgs A;
+    dev::SweepArgs<R> A;
     A.nC = ctx->nC;
     A.nchunks = ctx->nchunk;
+    A.ntiles = ctx->ntiles;
+    A.tile_chunk = ctx->tile_chunk;
     A.chunk = ctx->chunk;
     A.cfs = ctx->cfs;
     A.fcell = ctx->fcell;
-    A.fgeo = ctx->fgeo;
-    A.q = ctx->q[ctx->cur];
+    if constexpr (sizeof(R) == sizeof(double)) {
+        A.fgeo = ctx->fgeo;
+        A.q = ctx->q[ctx->cur];
+        A.diag = ctx->diag;
+        A.dqf = ctx->dqf;
+        A.dqb = ctx->dq;
+    } else {
+        A.fgeo = ctx->fgeo32;
+        A.q = ctx->q32;
+        A.diag = ctx->diag32;
+        A.dqf = ctx->dqf32;
+        A.dqb = ctx->dq32;
+    }
+    A.q64 = ctx->q[ctx->cur];
     A.res = ctx->res;
-    A.diag = ctx->diag;
-    A.dqf = ctx->dqf;
-    A.dqb = ctx->dq;
     A.qout = ctx->q[1 - ctx->cur];
     A.err = ctx->err;
     A.depmask = ctx->depmask;
     A.cell0 = ctx->cell0;
-    const dim3 block(kSweepWave);
-    const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.nchunks));
+    const dim3 block(dev::kSweepBlock);
+    const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.ntiles));
     {
         Timed t(ctx, "fwd_sweep");
         A.rep = ctx->rep_f;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<true, LPC>, grid, block, args, 0, ctx->stream),
+                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0,
+                                               ctx->stream),
                     "k_sweep<fwd>"))
             return false;
     }
@@ -242,40 +269,69 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
         A.rep = ctx->rep_b;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<false, LPC>, grid, block, args, 0, ctx->stream),
+                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<false, LPC, R>, grid, block, args, 0,
+                                               ctx->stream),
                     "k_sweep<bwd>"))
             return false;
     }
     ctx->cur = 1 - ctx->cur;
     return true;
 }
-bool launch_sweeps(mfs_ctx* ctx) { return ctx->lpc == 4 ? launch_sweeps_t<4>(ctx) : launch_sweeps_t<8>(ctx); }
+bool launch_sweeps(mfs_ctx* ctx) {
+    if (ctx->precision == MFS_MIXED)
+        return ctx->lpc == 4 ? launch_sweeps_t<4, float>(ctx) : launch_sweeps_t<8, float>(ctx);
+    return ctx->lpc == 4 ? launch_sweeps_t<4, double>(ctx) : launch_sweeps_t<8, double>(ctx);
+}
 
-template <int MAXF>
+template <int MAXF, typename R>
 bool launch_gradient_t(mfs_ctx* ctx) {
     const int n = ctx->nC + ctx->nB;
     const double* q = ctx->q[ctx->cur];
+    constexpr bool F64 = sizeof(R) == sizeof(double);
+    R* qn;
+    const R* nlw;
+    const typename dev::Real<R>::vec4* fgeo;
+    R *gx, *gy, *gz;
+    if constexpr (F64) {
+        qn = ctx->qn;
+        nlw = ctx->nlw;
+        fgeo = ctx->fgeo;
+        gx = ctx->gx;
+        gy = ctx->gy;
+        gz = ctx->gz;
+    } else {
+        qn = ctx->qn32;
+        nlw = ctx->nlw32;
+        fgeo = ctx->fgeo32;
+        gx = ctx->gx32;
+        gy = ctx->gy32;
+        gz = ctx->gz32;
+    }
     {
         Timed t(ctx, "node");
-        hipLaunchKernelGGL(dev::k_node, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo, ctx->nlab,
-                           ctx->nlw, ctx->nden, q, ctx->qn);
+        hipLaunchKernelGGL(dev::k_node<R>, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo,
+                           ctx->nlab, nlw, ctx->nden, q, qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
-            hipLaunchKernelGGL(dev::k_node_symm, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, ctx->stream, ctx->nsym,
-                               ctx->sym_node, ctx->sym_n, ctx->qn);
+            hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, ctx->stream, ctx->nsym,
+                               ctx->sym_node, ctx->sym_n, qn);
             if (!check_launch(ctx, "k_node_symm")) return false;
         }
     }
     {
         Timed t(ctx, "gradient");
-        hipLaunchKernelGGL((dev::k_grad<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n, ctx->cfo,
-                           ctx->cf, ctx->fcell, ctx->fgeo, ctx->fno, ctx->fn, ctx->btype, q, ctx->qn, ctx->vol, ctx->gx,
-                           ctx->gy, ctx->gz);
+        hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
+                           ctx->cfo, ctx->cf, ctx->fcell, fgeo, ctx->fno, ctx->fn, ctx->btype, q, qn, ctx->vol, gx, gy,
+                           gz);
         if (!check_launch(ctx, "k_grad")) return false;
     }
     return true;
 }
-bool launch_gradient(mfs_ctx* ctx) { return ctx->maxf == 4 ? launch_gradient_t<4>(ctx) : launch_gradient_t<6>(ctx); }
+bool launch_gradient(mfs_ctx* ctx) {
+    if (ctx->precision == MFS_MIXED)
+        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx) : launch_gradient_t<6, float>(ctx);
+    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx) : launch_gradient_t<6, double>(ctx);
+}
 
 bool usable(mfs_ctx* ctx) {
     if (!ctx) {
@@ -306,15 +362,17 @@ void free_all(mfs_ctx* ctx) {
     ctx->pending.clear();
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     ctx->event_pool.clear();
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
     ctx->stream = nullptr;
 }
 
-template <int LPC>
+template <int LPC, typename R>
 int sweep_occupancy(int* per_cu) {
     int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, LPC>, kSweepWave, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, LPC>, kSweepWave, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, LPC, R>, dev::kSweepBlock, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, LPC, R>, dev::kSweepBlock, 0) !=
+            hipSuccess)
         return -1;
     *per_cu = std::min(a, b);
     return 0;
@@ -333,8 +391,8 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
         return MFS_ERR_ARG;
     }
     *out = nullptr;
-    if (precision != MFS_FP64) {
-        set_error("mfs_ctx_create: this build supports MFS_FP64 contexts only");
+    if (precision != MFS_FP64 && precision != MFS_MIXED) {
+        set_error("mfs_ctx_create: precision must be MFS_FP64 or MFS_MIXED");
         return MFS_ERR_ARG;
     }
     HostLayout L;
@@ -376,6 +434,7 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ctx->K.betq = 1.0 + ctx->K.alpq * ctx->K.phiq;
     ctx->nlev = L.nlev;
     ctx->nchunk = (int)(L.chunk.size() / 2);
+    ctx->ntiles = (int)L.tile_chunk.size() - 1;
     ctx->lpc = L.lpc;
     ctx->nblk_cell = (int)nblk_xcd(L.nC);
     ctx->node_of_dev = L.node_of_dev;
@@ -400,18 +459,30 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ok = ok && dalloc(ctx, ctx->sym_node, L.sym_node.size(), "sym_node") &&
          dalloc(ctx, ctx->sym_n, L.sym_n.size(), "sym_n");
     ok = ok && dalloc(ctx, ctx->chunk, (size_t)ctx->nchunk, "chunk") && dalloc(ctx, ctx->cfs, L.cfs.size(), "cfs") &&
+         dalloc(ctx, ctx->tile_chunk, L.tile_chunk.size(), "tile_chunk") &&
          dalloc(ctx, ctx->rep_f, L.rep_f.size(), "rep_f") &&
          dalloc(ctx, ctx->rep_b, L.rep_b.size(), "rep_b") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
+    const bool mixed = precision == MFS_MIXED;
+    if (mixed) {
+        ok = ok && dalloc(ctx, ctx->q32, 5 * n, "q32") && dalloc(ctx, ctx->diag32, L.nC, "diag32") &&
+             dalloc(ctx, ctx->dqf32, 5 * (size_t)L.nC, "dq_forward32") && dalloc(ctx, ctx->dq32, 5 * n, "dq32") &&
+             dalloc(ctx, ctx->gx32, 5 * n, "dqdx32") && dalloc(ctx, ctx->gy32, 5 * n, "dqdy32") &&
+             dalloc(ctx, ctx->gz32, 5 * n, "dqdz32") && dalloc(ctx, ctx->qn32, 5 * (size_t)L.nN, "qnode32") &&
+             dalloc(ctx, ctx->nlw32, L.nlw.size(), "nlw32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32");
+    }
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
         // co-resident waves for the cooperative sweep launch
         int per_cu = 0, cus = 0;
-        ok = (ctx->lpc == 4 ? sweep_occupancy<4>(&per_cu) : sweep_occupancy<8>(&per_cu)) == 0 &&
+        const bool mixed = ctx->precision == MFS_MIXED;
+        const int occ = ctx->lpc == 4 ? (mixed ? sweep_occupancy<4, float>(&per_cu) : sweep_occupancy<4, double>(&per_cu))
+                                      : (mixed ? sweep_occupancy<8, float>(&per_cu) : sweep_occupancy<8, double>(&per_cu));
+        ok = occ == 0 &&
              hip_ok(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
         if (!ok) set_error("mfs_ctx_create: occupancy query failed");
-        ctx->sweep_grid = std::max(1, std::min(per_cu, 16) * cus);
+        ctx->sweep_grid = std::max(1, per_cu * cus);  // co-resident workgroups
     }
     if (ok) {
         std::vector<int2> fc(L.nF);
@@ -440,17 +511,48 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
              upload(ctx, ctx->chunk, reinterpret_cast<const int2*>(L.chunk.data()), (size_t)ctx->nchunk, "upload chunk") &&
              upload(ctx, ctx->cfs, L.cfs.data(), L.cfs.size(), "upload cfs") &&
+             upload(ctx, ctx->tile_chunk, L.tile_chunk.data(), L.tile_chunk.size(), "upload tile_chunk") &&
              upload(ctx, ctx->rep_f, L.rep_f.data(), L.rep_f.size(), "upload rep_f") &&
              upload(ctx, ctx->rep_b, L.rep_b.data(), L.rep_b.size(), "upload rep_b") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask");
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
+        std::vector<float> f32;
+        std::vector<float4> g32;
+        if (ok && mixed) {
+            // FP32 copies of the operator's static inputs; the ghost part of
+            // the state snapshot is fixed (ghosts are frozen after set-up)
+            g32.resize(L.nF);
+            for (int i = 0; i < L.nF; ++i)
+                g32[i] = make_float4((float)L.fgeo[4 * (size_t)i], (float)L.fgeo[4 * (size_t)i + 1],
+                                     (float)L.fgeo[4 * (size_t)i + 2], (float)L.fgeo[4 * (size_t)i + 3]);
+            f32.resize(std::max(L.nlw.size(), 5 * n));
+            for (size_t k = 0; k < L.nlw.size(); ++k) f32[k] = (float)L.nlw[k];
+            ok = upload(ctx, ctx->fgeo32, g32.data(), L.nF, "upload fgeo32") &&
+                 upload(ctx, ctx->nlw32, f32.data(), L.nlw.size(), "upload nlw32");
+            for (size_t k = 0; k < 5 * n; ++k) f32[k] = (float)qa[k];
+            ok = ok && upload(ctx, ctx->q32, f32.data(), 5 * n, "upload q32");
+            // hipMemcpyAsync reads the host buffers later: wait before they go out of scope
+            ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "upload sync");
+        }
         if (ok) {
             double sentinel;
             std::memcpy(&sentinel, &dev::kSentinel, sizeof(sentinel));
             const size_t cnt = 5 * (size_t)L.nC;
             hipLaunchKernelGGL(dev::k_fill, dim3(nblk(cnt)), dim3(kBlock), 0, ctx->stream, ctx->dqf, cnt, sentinel);
             ok = check_launch(ctx, "k_fill(dq_forward)");
+            if (ok && mixed) {
+                float sent32;
+                const unsigned bits = dev::Real<float>::kSent;
+                std::memcpy(&sent32, &bits, sizeof(sent32));
+                hipLaunchKernelGGL(dev::k_fill_f, dim3(nblk(cnt)), dim3(kBlock), 0, ctx->stream, ctx->dqf32, cnt,
+                                   sent32);
+                ok = check_launch(ctx, "k_fill(dq_forward32)") &&
+                     hip_ok(ctx, hipMemsetAsync(ctx->dq32, 0, sizeof(float) * 5 * n, ctx->stream), "memset dq32") &&
+                     hip_ok(ctx, hipMemsetAsync(ctx->gx32, 0, sizeof(float) * 5 * n, ctx->stream), "memset dqdx32") &&
+                     hip_ok(ctx, hipMemsetAsync(ctx->gy32, 0, sizeof(float) * 5 * n, ctx->stream), "memset dqdy32") &&
+                     hip_ok(ctx, hipMemsetAsync(ctx->gz32, 0, sizeof(float) * 5 * n, ctx->stream), "memset dqdz32");
+            }
         }
         ok = ok && hip_ok(ctx, hipMemsetAsync(ctx->err, 0, sizeof(int) * 4, ctx->stream), "memset err") &&
              hip_ok(ctx, hipMemsetAsync(ctx->dq, 0, sizeof(double) * 5 * n, ctx->stream), "memset dq") &&
@@ -508,6 +610,7 @@ extern "C" int mfs_compute_timestep(mfs_ctx* ctx) {
 extern "C" int mfs_forward_lusgs(mfs_ctx* ctx) {
     if (!usable(ctx)) return MFS_ERR_STATE;
     if (!launch_diag(ctx, false)) return MFS_ERR_HIP;
+    if (ctx->precision == MFS_MIXED && !launch_mixed_inputs(ctx)) return MFS_ERR_HIP;
     return launch_sweeps(ctx) ? MFS_OK : MFS_ERR_HIP;
 }
 
@@ -599,6 +702,7 @@ struct FieldRef {
     size_t rows = 0;    // cells or nodes for cell/node-major arrays
     FieldKind kind = FieldKind::Scalar;
     int block = 0;      // state variable for StateBlock
+    float* dev32 = nullptr;  // FP32 array of a mixed-precision context (read-only fields)
 };
 FieldRef field_ref(mfs_ctx* ctx, int field) {
     const size_t n = (size_t)ctx->nC + ctx->nB;
@@ -621,6 +725,17 @@ FieldRef field_ref(mfs_ctx* ctx, int field) {
         case MFS_F_QNODE: f = {ctx->qn, 5 * (size_t)ctx->nN, (size_t)ctx->nN, FieldKind::NodeMajor5, 0}; break;
         default: break;
     }
+    if (ctx->precision == MFS_MIXED) {
+        // the implicit operator's results and the gradients live in FP32
+        switch (field) {
+            case MFS_F_DQ: f.dev32 = ctx->dq32; break;
+            case MFS_F_DQDX: f.dev32 = ctx->gx32; break;
+            case MFS_F_DQDY: f.dev32 = ctx->gy32; break;
+            case MFS_F_DQDZ: f.dev32 = ctx->gz32; break;
+            case MFS_F_QNODE: f.dev32 = ctx->qn32; break;
+            default: break;
+        }
+    }
     return f;
 }
 }  // namespace
@@ -640,8 +755,15 @@ extern "C" int mfs_download(mfs_ctx* ctx, int field, double* dst, size_t count) 
     int r = mfs_sync(ctx);
     if (r != MFS_OK) return r;
     std::vector<double> tmp(f.kind == FieldKind::Scalar ? f.rows : 5 * f.rows);
-    if (!hip_ok(ctx, hipMemcpy(tmp.data(), f.dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost), "download"))
+    if (f.dev32) {
+        std::vector<float> t32(tmp.size());
+        if (!hip_ok(ctx, hipMemcpy(t32.data(), f.dev32, sizeof(float) * t32.size(), hipMemcpyDeviceToHost), "download"))
+            return MFS_ERR_HIP;
+        for (size_t k = 0; k < t32.size(); ++k) tmp[k] = (double)t32[k];
+    } else if (!hip_ok(ctx, hipMemcpy(tmp.data(), f.dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost),
+                       "download")) {
         return MFS_ERR_HIP;
+    }
     // device row -> mesh row: cells through cell_of_dev, ghosts unchanged, nodes through node_of_dev
     const size_t nC = (size_t)ctx->nC;
     auto mesh_row = [&](size_t d) -> size_t {
@@ -687,11 +809,17 @@ extern "C" int mfs_upload(mfs_ctx* ctx, int field, const double* src, size_t cou
             return MFS_ERR_HIP;
         for (size_t d = 0; d < f.rows; ++d) tmp[d * 5 + f.block] = src[mesh_row(d)];
         // both state buffers, so the ghost values stay identical in each
-        return hip_ok(ctx, hipMemcpy(ctx->q[0], tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice), "upload") &&
-                       hip_ok(ctx, hipMemcpy(ctx->q[1], tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice),
-                              "upload")
-                   ? MFS_OK
-                   : MFS_ERR_HIP;
+        bool ok = hip_ok(ctx, hipMemcpy(ctx->q[0], tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice),
+                         "upload") &&
+                  hip_ok(ctx, hipMemcpy(ctx->q[1], tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice),
+                         "upload");
+        if (ok && ctx->precision == MFS_MIXED) {  // the FP32 snapshot, ghosts included
+            std::vector<float> t32(tmp.size());
+            for (size_t k = 0; k < tmp.size(); ++k) t32[k] = (float)tmp[k];
+            ok = hip_ok(ctx, hipMemcpy(ctx->q32, t32.data(), sizeof(float) * t32.size(), hipMemcpyHostToDevice),
+                        "upload");
+        }
+        return ok ? MFS_OK : MFS_ERR_HIP;
     } else {
         for (size_t d = 0; d < f.rows; ++d)
             for (int j = 0; j < 5; ++j) tmp[d * 5 + j] = src[(size_t)j * f.rows + mesh_row(d)];
@@ -753,4 +881,439 @@ extern "C" int mfs_ctx_reset_timing(mfs_ctx* ctx) {
     int r = mfs_sync(ctx);
     ctx->timers.clear();
     return r;
+}
+
+// ===========================================================================
+// Batched multi-solve (mfs_batch_*): several solves stepped together, one
+// launch per kernel for all of them, on the batch's stream.
+struct mfs_batch {
+    int device = 0;
+    int precision = MFS_FP64;
+    int maxf = 0, lpc = 0;
+    hipStream_t stream = nullptr;
+    std::vector<mfs_ctx*> slots;  // removed solves leave nullptr
+    bool dirty = true;            // block maps and sweep items need a rebuild
+    // device-side maps for the current set of solves (position = index in `active`)
+    std::vector<int> active;      // slot of each position
+    int ncell_blk = 0, nnode_blk = 0, nitems = 0;
+    int* cell_solve = nullptr;
+    int* node_solve = nullptr;
+    int2* items_f = nullptr;
+    int2* items_b = nullptr;
+    size_t maps_cap = 0;
+    // per-iteration solve tables
+    void* table_host = nullptr;   // pinned
+    void* table_dev = nullptr;
+    size_t table_cap = 0;         // bytes
+    hipEvent_t table_copied = nullptr;
+    bool table_pending = false;
+    int sweep_grid = 0;
+};
+
+namespace {
+
+template <typename R>
+size_t batch_entry_bytes() {
+    return sizeof(dev::BatchSolve<R>);
+}
+
+bool batch_ok(mfs_batch* b, hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+bool batch_rebuild_maps(mfs_batch* b) {
+    b->active.clear();
+    for (size_t i = 0; i < b->slots.size(); ++i)
+        if (b->slots[i]) b->active.push_back((int)i);
+    std::vector<int> cs, ns;
+    std::vector<int2> itf, itb;
+    for (size_t a = 0; a < b->active.size(); ++a) {
+        mfs_ctx* c = b->slots[b->active[a]];
+        cs.insert(cs.end(), (size_t)c->nblk_cell, (int)a);
+        ns.insert(ns.end(), (size_t)nblk_xcd(c->nN), (int)a);
+        for (int k = 0; k < c->ntiles; ++k) {
+            itf.push_back(make_int2((int)a, k));
+            itb.push_back(make_int2((int)a, c->ntiles - 1 - k));
+        }
+    }
+    b->ncell_blk = (int)cs.size();
+    b->nnode_blk = (int)ns.size();
+    b->nitems = (int)itf.size();
+    if (!batch_ok(b, hipStreamSynchronize(b->stream), "batch sync")) return false;
+    const size_t need = std::max({cs.size(), ns.size(), itf.size(), (size_t)1});
+    if (need > b->maps_cap) {
+        (void)hipFree(b->cell_solve);
+        (void)hipFree(b->node_solve);
+        (void)hipFree(b->items_f);
+        (void)hipFree(b->items_b);
+        b->cell_solve = b->node_solve = nullptr;
+        b->items_f = b->items_b = nullptr;
+        const size_t cap = need + need / 2;
+        if (!batch_ok(b, hipMalloc((void**)&b->cell_solve, cap * sizeof(int)), "hipMalloc(batch maps)") ||
+            !batch_ok(b, hipMalloc((void**)&b->node_solve, cap * sizeof(int)), "hipMalloc(batch maps)") ||
+            !batch_ok(b, hipMalloc((void**)&b->items_f, cap * sizeof(int2)), "hipMalloc(batch maps)") ||
+            !batch_ok(b, hipMalloc((void**)&b->items_b, cap * sizeof(int2)), "hipMalloc(batch maps)"))
+            return false;
+        b->maps_cap = cap;
+    }
+    if (!cs.empty() &&
+        (!batch_ok(b, hipMemcpy(b->cell_solve, cs.data(), cs.size() * sizeof(int), hipMemcpyHostToDevice), "maps") ||
+         !batch_ok(b, hipMemcpy(b->node_solve, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice), "maps") ||
+         !batch_ok(b, hipMemcpy(b->items_f, itf.data(), itf.size() * sizeof(int2), hipMemcpyHostToDevice), "maps") ||
+         !batch_ok(b, hipMemcpy(b->items_b, itb.data(), itb.size() * sizeof(int2), hipMemcpyHostToDevice), "maps")))
+        return false;
+    b->dirty = false;
+    return true;
+}
+
+// The table entry of one solve for one iteration: `cur` is the state buffer
+// the iteration starts from.
+template <typename R>
+void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int blk0_node, int iter_done, int slot) {
+    std::memset(&E, 0, sizeof(E));
+    const bool f64 = sizeof(R) == sizeof(double);
+    E.nC = c->nC;
+    E.nN = c->nN;
+    E.n = c->nC + c->nB;
+    E.blk0_cell = blk0_cell;
+    E.nblk_cell = c->nblk_cell;
+    E.blk0_node = blk0_node;
+    E.nblk_node = (int)nblk_xcd(c->nN);
+    E.cfo = c->cfo;
+    E.cf = c->cf;
+    E.fcell = c->fcell;
+    E.fgeo = c->fgeo;
+    E.fdn = c->fdn;
+    E.vol = c->vol;
+    E.q = c->q[cur];
+    E.qnew = c->q[1 - cur];
+    E.res = c->res;
+    E.dt = c->dt;
+    E.diag = c->diag;
+    E.part = c->part;
+    E.run = c->run;
+    E.hist = c->hist;
+    E.hist_slot = slot;
+    E.cfl = cfl_of_iter(c->P, iter_done);
+    E.cfl_min = 0.5 * E.cfl;
+    E.ratio_dtmax = c->P.ratio_dtmax;
+    E.q32 = c->q32;
+    E.diag32 = c->diag32;
+    E.K = c->K;
+    E.nlo = c->nlo;
+    E.nlab = c->nlab;
+    E.nden = c->nden;
+    E.fno = c->fno;
+    E.fn = c->fn;
+    E.btype = c->btype;
+    auto& A = E.sw;
+    A.nC = c->nC;
+    A.nchunks = c->nchunk;
+    A.ntiles = c->ntiles;
+    A.tile_chunk = c->tile_chunk;
+    A.chunk = c->chunk;
+    A.cfs = c->cfs;
+    A.fcell = c->fcell;
+    A.q64 = c->q[cur];
+    A.res = c->res;
+    A.qout = c->q[1 - cur];
+    A.rep = nullptr;  // set per sweep direction by the kernel launch below
+    A.depmask = c->depmask;
+    A.cell0 = c->cell0;
+    A.err = c->err;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        E.nlw = c->nlw;
+        E.qn = c->qn;
+        E.fgeo_r = c->fgeo;
+        E.gx = c->gx;
+        E.gy = c->gy;
+        E.gz = c->gz;
+        A.fgeo = c->fgeo;
+        A.q = c->q[cur];
+        A.diag = c->diag;
+        A.dqf = c->dqf;
+        A.dqb = c->dq;
+    } else {
+        E.nlw = c->nlw32;
+        E.qn = c->qn32;
+        E.fgeo_r = c->fgeo32;
+        E.gx = c->gx32;
+        E.gy = c->gy32;
+        E.gz = c->gz32;
+        A.fgeo = c->fgeo32;
+        A.q = c->q32;
+        A.diag = c->diag32;
+        A.dqf = c->dqf32;
+        A.dqb = c->dq32;
+    }
+    (void)f64;
+}
+
+template <int MAXF, int LPC, typename R>
+int batch_step_t(mfs_batch* b, int nsteps) {
+    constexpr bool MIXED = sizeof(R) == sizeof(float);
+    const int ns = (int)b->active.size();
+    if (ns == 0 || nsteps == 0) return MFS_OK;
+    for (int a = 0; a < ns; ++a) {
+        mfs_ctx* c = b->slots[b->active[a]];
+        if (c->broken) {
+            set_error("mfs_batch_step: a solve of the batch is in an error state");
+            return MFS_ERR_STATE;
+        }
+        if (!ensure_hist(c, c->hist_n + nsteps)) return MFS_ERR_HIP;
+    }
+    // two tables per iteration (forward and backward sweep reps differ)
+    const size_t per_iter = 2 * (size_t)ns * sizeof(dev::BatchSolve<R>);
+    const size_t bytes = per_iter * (size_t)nsteps;
+    if (b->table_pending) {
+        if (!batch_ok(b, hipEventSynchronize(b->table_copied), "batch table event")) return MFS_ERR_HIP;
+        b->table_pending = false;
+    }
+    if (bytes > b->table_cap) {
+        if (!batch_ok(b, hipStreamSynchronize(b->stream), "batch sync")) return MFS_ERR_HIP;
+        (void)hipHostFree(b->table_host);
+        (void)hipFree(b->table_dev);
+        b->table_host = b->table_dev = nullptr;
+        b->table_cap = 0;
+        const size_t cap = bytes + bytes / 2;
+        if (!batch_ok(b, hipHostMalloc(&b->table_host, cap), "hipHostMalloc(batch table)") ||
+            !batch_ok(b, hipMalloc(&b->table_dev, cap), "hipMalloc(batch table)"))
+            return MFS_ERR_NOMEM;
+        b->table_cap = cap;
+    }
+    auto* H = static_cast<dev::BatchSolve<R>*>(b->table_host);
+    for (int i = 0; i < nsteps; ++i) {
+        int blk_c = 0, blk_n = 0;
+        for (int a = 0; a < ns; ++a) {
+            mfs_ctx* c = b->slots[b->active[a]];
+            const int cur = c->cur ^ (i & 1);
+            dev::BatchSolve<R>& Ef = H[(size_t)i * 2 * ns + a];
+            fill_entry<R>(Ef, c, cur, blk_c, blk_n, c->iter_done + i, c->hist_n + i);
+            Ef.sw.rep = c->rep_f;
+            dev::BatchSolve<R>& Eb = H[(size_t)i * 2 * ns + ns + a];
+            Eb = Ef;
+            Eb.sw.rep = c->rep_b;
+            blk_c += c->nblk_cell;
+            blk_n += (int)nblk_xcd(c->nN);
+        }
+    }
+    if (!batch_ok(b, hipMemcpyAsync(b->table_dev, b->table_host, bytes, hipMemcpyHostToDevice, b->stream),
+                  "batch table upload") ||
+        !batch_ok(b, hipEventRecord(b->table_copied, b->stream), "batch table event"))
+        return MFS_ERR_HIP;
+    b->table_pending = true;
+    auto* D = static_cast<dev::BatchSolve<R>*>(b->table_dev);
+    const dim3 sweep_grid((unsigned)std::min(b->sweep_grid, b->nitems));
+    for (int i = 0; i < nsteps; ++i) {
+        const dev::BatchSolve<R>* Tf = D + (size_t)i * 2 * ns;
+        const dev::BatchSolve<R>* Tb = Tf + ns;
+        hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
+                           b->cell_solve);
+        hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(kBlock), 0, b->stream, Tf);
+        hipLaunchKernelGGL((dev::k_b_clamp<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
+                           b->cell_solve);
+        if (!batch_ok(b, hipGetLastError(), "batch flux/time step")) return MFS_ERR_HIP;
+        {
+            int n_it = b->nitems;
+            void* args[] = {(void*)&Tf, (void*)&b->items_f, (void*)&n_it};
+            if (!batch_ok(b,
+                          hipLaunchCooperativeKernel((const void*)dev::k_b_sweep<true, LPC, R>, sweep_grid,
+                                                     dim3(dev::kSweepBlock), args, 0, b->stream),
+                          "batch forward sweep"))
+                return MFS_ERR_HIP;
+            void* argsb[] = {(void*)&Tb, (void*)&b->items_b, (void*)&n_it};
+            if (!batch_ok(b,
+                          hipLaunchCooperativeKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
+                                                     dim3(dev::kSweepBlock), argsb, 0, b->stream),
+                          "batch backward sweep"))
+                return MFS_ERR_HIP;
+        }
+        hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, b->stream, Tf, b->node_solve);
+        for (int a = 0; a < ns; ++a) {  // symmetry-plane projection, only where a solve has SYMM nodes
+            mfs_ctx* c = b->slots[b->active[a]];
+            if (c->nsym > 0)
+                hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(c->nsym)), dim3(kBlock), 0, b->stream, c->nsym,
+                                   c->sym_node, c->sym_n, MIXED ? (R*)c->qn32 : (R*)c->qn);
+        }
+        hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
+                           b->cell_solve);
+        if (!batch_ok(b, hipGetLastError(), "batch gradient")) return MFS_ERR_HIP;
+    }
+    for (int a = 0; a < ns; ++a) {
+        mfs_ctx* c = b->slots[b->active[a]];
+        const double cfl = cfl_of_iter(c->P, c->iter_done + nsteps - 1);
+        c->cfl_min = 0.5 * cfl;
+        c->cur ^= (nsteps & 1);
+        c->hist_n += nsteps;
+        c->iter_done += nsteps;
+    }
+    return MFS_OK;
+}
+
+template <int LPC, typename R>
+int batch_sweep_occupancy(int* per_cu) {
+    int a = 0, c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_b_sweep<true, LPC, R>, dev::kSweepBlock, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, dev::k_b_sweep<false, LPC, R>, dev::kSweepBlock, 0) !=
+            hipSuccess)
+        return -1;
+    *per_cu = std::min(a, c);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int mfs_batch_create(int precision, mfs_batch** out) {
+    if (!out || (precision != MFS_FP64 && precision != MFS_MIXED)) {
+        set_error("mfs_batch_create: null argument or unknown precision");
+        return MFS_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_error("mfs_batch_create: no HIP device available");
+        return MFS_ERR_HIP;
+    }
+    mfs_batch* b = new mfs_batch();
+    b->precision = precision;
+    (void)hipGetDevice(&b->device);
+    if (!batch_ok(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+        !batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate")) {
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return MFS_ERR_HIP;
+    }
+    *out = b;
+    return MFS_OK;
+}
+
+extern "C" int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_params* p, int* slot) {
+    if (!b || !mesh || !p || !slot) {
+        set_error("mfs_batch_add: null argument");
+        return MFS_ERR_ARG;
+    }
+    if (hipSetDevice(b->device) != hipSuccess) {
+        set_error("mfs_batch_add: hipSetDevice failed");
+        return MFS_ERR_HIP;
+    }
+    mfs_ctx* c = nullptr;
+    int r = mfs_ctx_create(mesh, p, b->precision, &c);
+    if (r != MFS_OK) return r;
+    const bool first = b->maxf == 0;
+    if (!first && (c->maxf != b->maxf || c->lpc != b->lpc)) {
+        (void)mfs_ctx_destroy(c);
+        set_error("mfs_batch_add: a batch holds either tetrahedral meshes or meshes with up to 6 faces per cell, not both");
+        return MFS_ERR_MESH;
+    }
+    if (first) {
+        b->maxf = c->maxf;
+        b->lpc = c->lpc;
+        int per_cu = 0, cus = 0;
+        const bool mixed = b->precision == MFS_MIXED;
+        const int occ = b->lpc == 4 ? (mixed ? batch_sweep_occupancy<4, float>(&per_cu)
+                                             : batch_sweep_occupancy<4, double>(&per_cu))
+                                    : (mixed ? batch_sweep_occupancy<8, float>(&per_cu)
+                                             : batch_sweep_occupancy<8, double>(&per_cu));
+        if (occ != 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) {
+            (void)mfs_ctx_destroy(c);
+            set_error("mfs_batch_add: occupancy query failed");
+            return MFS_ERR_HIP;
+        }
+        b->sweep_grid = std::max(1, per_cu * cus);
+    }
+    // the solve's work goes to the batch stream from now on
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    c->stream = b->stream;
+    c->owns_stream = false;
+    int s = -1;
+    for (size_t i = 0; i < b->slots.size(); ++i)
+        if (!b->slots[i]) {
+            s = (int)i;
+            break;
+        }
+    if (s < 0) {
+        s = (int)b->slots.size();
+        b->slots.push_back(nullptr);
+    }
+    b->slots[s] = c;
+    b->dirty = true;
+    *slot = s;
+    return MFS_OK;
+}
+
+extern "C" int mfs_batch_remove(mfs_batch* b, int slot) {
+    if (!b || slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot]) {
+        set_error("mfs_batch_remove: no such solve");
+        return MFS_ERR_ARG;
+    }
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    int r = mfs_ctx_destroy(b->slots[slot]);
+    b->slots[slot] = nullptr;
+    b->dirty = true;
+    return r;
+}
+
+extern "C" int mfs_batch_step(mfs_batch* b, int nsteps) {
+    if (!b || nsteps < 0) {
+        set_error("mfs_batch_step: null batch or negative step count");
+        return MFS_ERR_ARG;
+    }
+    if (hipSetDevice(b->device) != hipSuccess) {
+        set_error("mfs_batch_step: hipSetDevice failed");
+        return MFS_ERR_HIP;
+    }
+    if (b->dirty && !batch_rebuild_maps(b)) return MFS_ERR_HIP;
+    if (b->precision == MFS_MIXED)
+        return b->lpc == 4 ? batch_step_t<4, 4, float>(b, nsteps) : batch_step_t<6, 8, float>(b, nsteps);
+    return b->lpc == 4 ? batch_step_t<4, 4, double>(b, nsteps) : batch_step_t<6, 8, double>(b, nsteps);
+}
+
+extern "C" int mfs_batch_sync(mfs_batch* b) {
+    if (!b) return MFS_ERR_ARG;
+    (void)hipSetDevice(b->device);
+    if (!batch_ok(b, hipStreamSynchronize(b->stream), "hipStreamSynchronize")) return MFS_ERR_HIP;
+    for (mfs_ctx* c : b->slots) {
+        if (!c) continue;
+        int r = mfs_sync(c);
+        if (r != MFS_OK) return r;
+    }
+    return MFS_OK;
+}
+
+extern "C" mfs_ctx* mfs_batch_ctx(mfs_batch* b, int slot) {
+    if (!b || slot < 0 || slot >= (int)b->slots.size()) return nullptr;
+    return b->slots[slot];
+}
+
+extern "C" int mfs_batch_count(mfs_batch* b) {
+    if (!b) return 0;
+    int n = 0;
+    for (mfs_ctx* c : b->slots) n += c ? 1 : 0;
+    return n;
+}
+
+extern "C" int mfs_batch_destroy(mfs_batch* b) {
+    if (!b) return MFS_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    for (mfs_ctx*& c : b->slots)
+        if (c) {
+            (void)mfs_ctx_destroy(c);
+            c = nullptr;
+        }
+    (void)hipFree(b->cell_solve);
+    (void)hipFree(b->node_solve);
+    (void)hipFree(b->items_f);
+    (void)hipFree(b->items_b);
+    (void)hipHostFree(b->table_host);
+    (void)hipFree(b->table_dev);
+    (void)hipEventDestroy(b->table_copied);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return MFS_OK;
 }

@@ -13,6 +13,24 @@
 // first face, so every load of a cell can be issued at once, and their
 // contributions are skipped.
 
+// Real types of the kernels: FP64 (the reference's arithmetic) and FP32
+// (the mixed-precision operator).  kSent is a NaN payload that arithmetic
+// never produces, used by the sweeps to mark unpublished values.
+template <typename R>
+struct Real;
+template <>
+struct Real<double> {
+    using bits = unsigned long long;
+    using vec4 = double4;
+    static constexpr bits kSent = 0x7FF4D51F5EED0000ULL;
+};
+template <>
+struct Real<float> {
+    using bits = unsigned int;
+    using vec4 = float4;
+    static constexpr bits kSent = 0x7FA5D51Fu;
+};
+
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
     __shared__ double s[kBlock / 64][NV];
@@ -75,6 +93,10 @@ __device__ __forceinline__ void load5(const double* __restrict__ p, double v[5])
 }
 
 __global__ void k_fill(double* p, size_t n, double v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f(float* p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -219,16 +241,16 @@ __global__ __launch_bounds__(kBlock) void k_timestep(int nC, const int* __restri
 // faces, for the fused iteration: the arithmetic of k_flux<false> and
 // k_timestep<true>, which both read the state at the start of the
 // iteration.  The cell's own state is loaded once and only the neighbour's
-// is gathered per face.
-template <int MAXF>
-__global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
-                                                    const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
-                                                    const double2* __restrict__ fdn, const double* __restrict__ q,
-                                                    const double* __restrict__ vol, double* __restrict__ res,
-                                                    double* __restrict__ dt, double* __restrict__ diag,
-                                                    double* __restrict__ part, FlowConst K, double cfl,
-                                                    double cfl_min) {
-    const int c = xcd_block() * kBlock + (int)threadIdx.x;
+// is gathered per face.  With MIXED the FP32 inputs of the implicit
+// operator (state snapshot, Diag) are written as well.
+template <int MAXF, bool MIXED>
+__device__ __forceinline__ void flux_ts_body(int c, int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                             const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                             const double2* __restrict__ fdn, const double* __restrict__ q,
+                                             const double* __restrict__ vol, double* __restrict__ res,
+                                             double* __restrict__ dt, double* __restrict__ diag,
+                                             double* __restrict__ pblk, const FlowConst& K, double cfl,
+                                             double cfl_min, float* __restrict__ q32, float* __restrict__ diag32) {
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double vmin = kBIGd, vmax = 0.0;
     if (c < nC) {
@@ -298,12 +320,30 @@ __global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restric
         for (int s = 0; s < MAXF; ++s)
             if (s < nf) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
         diag[c] = dg;
+        if (MIXED) {
+            diag32[c] = (float)dg;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) q32[(size_t)c * 5 + j] = (float)qc[j];
+        }
     }
     double s2[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
-    block_sum<5>(s2, part + (size_t)blockIdx.x * 8);
-    block_minmax(vmin, vmax, part + (size_t)blockIdx.x * 8 + 5, part + (size_t)blockIdx.x * 8 + 6);
+    block_sum<5>(s2, pblk);
+    block_minmax(vmin, vmax, pblk + 5, pblk + 6);
+}
+
+template <int MAXF, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                    const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                                    const double2* __restrict__ fdn, const double* __restrict__ q,
+                                                    const double* __restrict__ vol, double* __restrict__ res,
+                                                    double* __restrict__ dt, double* __restrict__ diag,
+                                                    double* __restrict__ part, FlowConst K, double cfl,
+                                                    double cfl_min, float* __restrict__ q32,
+                                                    float* __restrict__ diag32) {
+    flux_ts_body<MAXF, MIXED>(xcd_block() * kBlock + (int)threadIdx.x, nC, cfo, cf, fcell, fgeo, fdn, q, vol, res, dt,
+                              diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
 }
 
 // Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
@@ -311,12 +351,11 @@ __global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restric
 // and recomputed (compute_timestep.cpp:929-945), and only when the
 // reduction found the ratio exceeded.
 template <int MAXF>
-__global__ __launch_bounds__(kBlock) void k_diag(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
-                                                 const double4* __restrict__ fgeo, const double* __restrict__ q,
-                                                 const double* __restrict__ vol, double* __restrict__ dt,
-                                                 double* __restrict__ diag, const double* __restrict__ run,
-                                                 double ratio_dtmax, FlowConst K) {
-    const int c = xcd_block() * kBlock + (int)threadIdx.x;
+__device__ __forceinline__ void diag_body(int c, int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                          const double4* __restrict__ fgeo, const double* __restrict__ q,
+                                          const double* __restrict__ vol, double* __restrict__ dt,
+                                          double* __restrict__ diag, const double* __restrict__ run,
+                                          double ratio_dtmax, const FlowConst& K, float* __restrict__ diag32) {
     if (c >= nC) return;
     double dtc = dt[c];
     if (run) {
@@ -340,14 +379,36 @@ __global__ __launch_bounds__(kBlock) void k_diag(int nC, const int* __restrict__
         if (s < nf) dg += 0.5 * g.w * (vn + a) * K.lhs_omga;
     }
     diag[c] = dg;
+    if (diag32) diag32[c] = (float)dg;
+}
+template <int MAXF>
+__global__ __launch_bounds__(kBlock) void k_diag(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                 const double4* __restrict__ fgeo, const double* __restrict__ q,
+                                                 const double* __restrict__ vol, double* __restrict__ dt,
+                                                 double* __restrict__ diag, const double* __restrict__ run,
+                                                 double ratio_dtmax, FlowConst K, float* __restrict__ diag32) {
+    diag_body<MAXF>(xcd_block() * kBlock + (int)threadIdx.x, nC, cfo, cf, fgeo, q, vol, dt, diag, run, ratio_dtmax, K,
+                    diag32);
+}
+
+// FP32 operator inputs of a mixed-precision context from the FP64 state and
+// Diag (interior cells; the ghost part of the snapshot never changes).
+__global__ __launch_bounds__(kBlock) void k_mixed_inputs(int nC, const double* __restrict__ q,
+                                                         const double* __restrict__ diag, float* __restrict__ q32,
+                                                         float* __restrict__ diag32) {
+    const int c = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (c >= nC) return;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) q32[(size_t)c * 5 + j] = (float)q[(size_t)c * 5 + j];
+    diag32[c] = (float)diag[c];
 }
 
 // ---------------------------------------------------------------------------
 // Reductions.  mode bits: 1 = residual norms -> run[3..7], 2 = dt extrema
 // (running max/min of compute_timestep.cpp:914-918, clamp flag), 4 = write
 // history record `slot` from run[].
-__global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
-                                                   double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
+__device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__ part, double* __restrict__ run,
+                                            double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
     if (mode & 1) {
         double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int b = threadIdx.x; b < nblk; b += kBlock)
@@ -384,24 +445,28 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __res
         h[6] = run[1];
     }
 }
+__global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
+                                                   double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
+    reduce_body(nblk, part, run, hist, slot, mode, ratio_dtmax);
+}
 
 // ---------------------------------------------------------------------------
 // Node values (CompNodeVar3D_dist, cal_gradient.cpp:1922-2372): each node
 // sums its weighted contributions in the reference's order, then divides by
 // WeightNode + TINY.
-__global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
-                                                 const double* __restrict__ nlw, const double* __restrict__ nden,
-                                                 const double* __restrict__ q, double* __restrict__ qn) {
-    const int p = xcd_block() * kBlock + (int)threadIdx.x;
+template <typename R>
+__device__ __forceinline__ void node_body(int p, int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
+                                          const R* __restrict__ nlw, const double* __restrict__ nden,
+                                          const double* __restrict__ q, R* __restrict__ qn) {
     if (p >= nN) return;
-    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
     const int e0 = nlo[p], e1 = nlo[p + 1];
     // entries in groups of U: the gathers of a group are issued together,
     // then added in entry order
     constexpr int U = 6;
     for (int k = e0; k < e1; k += U) {
         int2 ab[U];
-        double w[U], qa[U][5];
+        R w[U], qa[U][5];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
             const int e = (k + i < e1) ? k + i : k;
@@ -409,7 +474,9 @@ __global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__
             w[i] = nlw[e];
         }
 #pragma unroll
-        for (int i = 0; i < U; ++i) load5(q + (size_t)ab[i].x * 5, qa[i]);
+        for (int i = 0; i < U; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) qa[i][j] = (R)q[(size_t)ab[i].x * 5 + j];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
             if (k + i >= e1) break;
@@ -417,31 +484,37 @@ __global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__
 #pragma unroll
                 for (int j = 0; j < 5; ++j) acc[j] += qa[i][j] * w[i];
             } else {
-                double qb[5];
-                load5(q + (size_t)ab[i].y * 5, qb);
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
-                    double fq = qa[i][j] + qb[j];
-                    fq *= 0.5;
+                    R fq = qa[i][j] + (R)q[(size_t)ab[i].y * 5 + j];
+                    fq *= R(0.5);
                     acc[j] += fq * w[i];
                 }
             }
         }
     }
-    const double d = nden[p];
+    const R d = (R)nden[p];
 #pragma unroll
     for (int j = 0; j < 5; ++j) qn[(size_t)p * 5 + j] = acc[j] / d;
 }
 
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
+                                                 const R* __restrict__ nlw, const double* __restrict__ nden,
+                                                 const double* __restrict__ q, R* __restrict__ qn) {
+    node_body<R>(xcd_block() * kBlock + (int)threadIdx.x, nN, nlo, nlab, nlw, nden, q, qn);
+}
+
 // Symmetry-plane velocity projection of the node values (cal_gradient.cpp:2332-2360).
+template <typename R>
 __global__ void k_node_symm(int nsym, const int* __restrict__ sym_node, const double* __restrict__ sym_n,
-                            double* __restrict__ qn) {
+                            R* __restrict__ qn) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsym) return;
     const size_t p = (size_t)sym_node[s] * 5;
-    const double xs = sym_n[3 * s], ys = sym_n[3 * s + 1], zs = sym_n[3 * s + 2];
-    const double un = qn[p + 1], vn = qn[p + 2], wn = qn[p + 3];
-    const double vv = un * xs + vn * ys + wn * zs;
+    const R xs = (R)sym_n[3 * s], ys = (R)sym_n[3 * s + 1], zs = (R)sym_n[3 * s + 2];
+    const R un = qn[p + 1], vn = qn[p + 2], wn = qn[p + 3];
+    const R vv = un * xs + vn * ys + wn * zs;
     qn[p + 1] = un - vv * xs;
     qn[p + 2] = vn - vv * ys;
     qn[p + 3] = wn - vv * zs;
@@ -451,21 +524,21 @@ __global__ void k_node_symm(int nsym, const int* __restrict__ sym_node, const do
 // 1889-1897): face values from the node values (interior faces, SYMM and
 // INTERFACE boundary faces) or the face-centre average of cell and ghost;
 // outputs in the reference's per-variable blocks of nC+nB.
-template <int MAXF>
-__global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __restrict__ cfo,
-                                                 const int* __restrict__ cf, const int2* __restrict__ fcell,
-                                                 const double4* __restrict__ fgeo, const int* __restrict__ fno,
-                                                 const int* __restrict__ fn, const int* __restrict__ btype,
-                                                 const double* __restrict__ q, const double* __restrict__ qn,
-                                                 const double* __restrict__ vol, double* __restrict__ gx,
-                                                 double* __restrict__ gy, double* __restrict__ gz) {
-    const int c = xcd_block() * kBlock + (int)threadIdx.x;
+template <int MAXF, typename R>
+__device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __restrict__ cfo,
+                                          const int* __restrict__ cf, const int2* __restrict__ fcell,
+                                          const typename Real<R>::vec4* __restrict__ fgeo,
+                                          const int* __restrict__ fno, const int* __restrict__ fn,
+                                          const int* __restrict__ btype, const double* __restrict__ q,
+                                          const R* __restrict__ qn, const double* __restrict__ vol,
+                                          R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz) {
     if (c >= nC) return;
-    double ax[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ay[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, az[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    R ax[5] = {R(0), R(0), R(0), R(0), R(0)}, ay[5] = {R(0), R(0), R(0), R(0), R(0)},
+      az[5] = {R(0), R(0), R(0), R(0), R(0)};
     int fs[MAXF];
     const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
     int2 cc[MAXF];
-    double4 g[MAXF];
+    typename Real<R>::vec4 g[MAXF];
     // tetrahedral meshes: every face is a triangle and its nodes sit at 3f
     int tn[MAXF == 4 ? 12 : 1];
 #pragma unroll
@@ -481,51 +554,50 @@ __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __res
     for (int s = 0; s < MAXF; ++s) {
         if (s >= nf) break;
         const int f = fs[s];
-        double qs[5];
+        R qs[5];
         bool node_mean = true;
         if (cc[s].y >= nC) {
             const int t = btype[cc[s].y - nC];
             node_mean = (t == BC_INTERFACE || t == BC_SYMM);
         }
         if (node_mean) {
-            double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            double cnt;
+            R sum[5] = {R(0), R(0), R(0), R(0), R(0)};
+            R cnt;
             if constexpr (MAXF == 4) {
-                double v0[5], v1[5], v2[5];
-                load5(qn + (size_t)tn[3 * s] * 5, v0);
-                load5(qn + (size_t)tn[3 * s + 1] * 5, v1);
-                load5(qn + (size_t)tn[3 * s + 2] * 5, v2);
+                R v0[5], v1[5], v2[5];
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    v0[j] = qn[(size_t)tn[3 * s] * 5 + j];
+                    v1[j] = qn[(size_t)tn[3 * s + 1] * 5 + j];
+                    v2[j] = qn[(size_t)tn[3 * s + 2] * 5 + j];
+                }
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
                     sum[j] += v0[j];
                     sum[j] += v1[j];
                     sum[j] += v2[j];
                 }
-                cnt = 3.0;
+                cnt = R(3.0);
             } else {
                 const int j0 = fno[f], j1 = fno[f + 1];
-                cnt = (double)(j1 - j0);
+                cnt = (R)(j1 - j0);
                 for (int m = j0; m < j1; ++m) {
-                    double v5[5];
-                    load5(qn + (size_t)fn[m] * 5, v5);
 #pragma unroll
-                    for (int j = 0; j < 5; ++j) sum[j] += v5[j];
+                    for (int j = 0; j < 5; ++j) sum[j] += qn[(size_t)fn[m] * 5 + j];
                 }
             }
 #pragma unroll
             for (int j = 0; j < 5; ++j) qs[j] = sum[j] / cnt;
         } else {
-            double qa[5], qb[5];
-            load5(q + (size_t)cc[s].x * 5, qa);
-            load5(q + (size_t)cc[s].y * 5, qb);
 #pragma unroll
-            for (int j = 0; j < 5; ++j) qs[j] = 0.5 * (qa[j] + qb[j]);
+            for (int j = 0; j < 5; ++j)
+                qs[j] = R(0.5) * ((R)q[(size_t)cc[s].x * 5 + j] + (R)q[(size_t)cc[s].y * 5 + j]);
         }
         const bool minus = (cc[s].y < nC) && (cc[s].y == c);
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-            const double sa = qs[j] * g[s].w;
-            const double tx = sa * g[s].x, ty = sa * g[s].y, tz = sa * g[s].z;
+            const R sa = qs[j] * g[s].w;
+            const R tx = sa * g[s].x, ty = sa * g[s].y, tz = sa * g[s].z;
             if (minus) {
                 ax[j] -= tx;
                 ay[j] -= ty;
@@ -537,11 +609,23 @@ __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __res
             }
         }
     }
-    const double vc = vol[c];
+    const R vc = (R)vol[c];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
         gx[(size_t)j * n + c] = ax[j] / vc;
         gy[(size_t)j * n + c] = ay[j] / vc;
         gz[(size_t)j * n + c] = az[j] / vc;
     }
+}
+
+template <int MAXF, typename R>
+__global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __restrict__ cfo,
+                                                 const int* __restrict__ cf, const int2* __restrict__ fcell,
+                                                 const typename Real<R>::vec4* __restrict__ fgeo,
+                                                 const int* __restrict__ fno, const int* __restrict__ fn,
+                                                 const int* __restrict__ btype, const double* __restrict__ q,
+                                                 const R* __restrict__ qn, const double* __restrict__ vol,
+                                                 R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz) {
+    grad_body<MAXF, R>(xcd_block() * kBlock + (int)threadIdx.x, nC, n, cfo, cf, fcell, fgeo, fno, fn, btype, q, qn,
+                       vol, gx, gy, gz);
 }

@@ -362,60 +362,66 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
 
     // ---- sweep chunks.  A chunk is up to `cpc` consecutive device cells of
     // one tile and one forward level, so its cells never depend on one
-    // another.  The forward sweep runs the chunks by level (then tile); the
-    // backward sweep runs them in the reverse order, which is valid for it
-    // because an upper neighbour always has a higher forward level.  `chunk`
-    // lists the chunks in forward run order; rep_f[u] is the dependency of
-    // forward chunk u in the latest chunk of the run order, rep_b[u] the same
-    // for the chunk the backward sweep runs u-th (forward chunk nchunk-1-u).
+    // another.  Chunks are numbered in device order (tile, then level), and
+    // tile k owns chunks tile_chunk[k] .. tile_chunk[k+1]-1.  A lower
+    // neighbour has a smaller mesh id (same or earlier tile) and a lower
+    // level, so every forward dependency lies in an earlier chunk and every
+    // backward dependency in a later one.  rep_f/rep_b name, per chunk, the
+    // dependency expected to be published last: the one with the highest
+    // (forward) or lowest (backward) level, another tile's first.
     L.lpc = L.max_faces <= 4 ? 4 : 8;
     L.cpc = 64 / L.lpc;
     {
-        struct Ch {
-            int level, start, count;
-        };
-        std::vector<Ch> chs;
+        std::vector<int> chunk_of(nC);
+        L.chunk.clear();
+        L.tile_chunk.clear();
+        int cur_tile = -1;
         for (int d = 0; d < nC;) {
             const int c0 = L.cell_of_dev[d];
             const int l = L.flevel[c0], tile = c0 / kTileCells;
+            while (cur_tile < tile) {
+                L.tile_chunk.push_back((int)(L.chunk.size() / 2));
+                ++cur_tile;
+            }
             int e = d;
             while (e < nC && e - d < L.cpc && L.flevel[L.cell_of_dev[e]] == l &&
                    L.cell_of_dev[e] / kTileCells == tile)
                 ++e;
-            chs.push_back({l, d, e - d});
+            const int t = (int)(L.chunk.size() / 2);
+            for (int x = d; x < e; ++x) chunk_of[x] = t;
+            L.chunk.push_back(d);
+            L.chunk.push_back(e - d);
             d = e;
         }
-        std::stable_sort(chs.begin(), chs.end(), [](const Ch& a, const Ch& b) { return a.level < b.level; });
-        const int nchunk = (int)chs.size();
-        std::vector<int> run_of(nC);
-        L.chunk.resize(2 * (size_t)nchunk);
-        for (int u = 0; u < nchunk; ++u) {
-            L.chunk[2 * u] = chs[u].start;
-            L.chunk[2 * u + 1] = chs[u].count;
-            for (int x = chs[u].start; x < chs[u].start + chs[u].count; ++x) run_of[x] = u;
-        }
+        const int nchunk = (int)(L.chunk.size() / 2);
+        L.tile_chunk.push_back(nchunk);
         L.rep_f.assign(nchunk, -1);
         L.rep_b.assign(nchunk, -1);
-        std::vector<int> bestF(nchunk, -1), bestB(nchunk, nchunk);
+        // rank of a dependency: level first, then "in another tile"
+        std::vector<long long> bestF(nchunk, -1), bestB(nchunk, -1);
         for (int d = 0; d < nC; ++d) {
             const int c = L.cell_of_dev[d];
-            const int t = run_of[d];
+            const int t = chunk_of[d];
             for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
                 const int f = mcf[k];
                 if (f < nB) continue;
                 const int nb = other(c, f);
-                const int tn = run_of[dcell[nb]];
+                const int dn = dcell[nb];
+                const int tn = chunk_of[dn];
+                const int foreign = (nb / kTileCells != c / kTileCells) ? 1 : 0;
                 if (nb < c) {
                     if (tn >= t) return std::string("forward sweep chunks are not topological");
-                    if (tn > bestF[t]) {
-                        bestF[t] = tn;
-                        L.rep_f[t] = dcell[nb];
+                    const long long r = 2LL * L.flevel[nb] + foreign;
+                    if (r > bestF[t]) {
+                        bestF[t] = r;
+                        L.rep_f[t] = dn;
                     }
                 } else {
                     if (tn <= t) return std::string("backward sweep chunks are not topological");
-                    if (tn < bestB[t]) {
-                        bestB[t] = tn;
-                        L.rep_b[nchunk - 1 - t] = dcell[nb];
+                    const long long r = 2LL * (L.nlev - L.flevel[nb]) + foreign;
+                    if (r > bestB[t]) {
+                        bestB[t] = r;
+                        L.rep_b[t] = dn;
                     }
                 }
             }

@@ -61,8 +61,9 @@ struct HostLayout {
     std::vector<int> flevel;                   // nC, by mesh id
     int nlev = 0;
     int lpc = 4, cpc = 16;
-    std::vector<int> chunk;                    // 2 per chunk: first device cell, cell count
-    std::vector<int> rep_f, rep_b;             // per chunk in run order: device cell published last (-1: none)
+    std::vector<int> chunk;                    // 2 per chunk (device order): first device cell, cell count
+    std::vector<int> tile_chunk;               // ntile+1: first chunk of each storage tile
+    std::vector<int> rep_f, rep_b;             // per chunk: dependency expected to be published last (-1: none)
     std::vector<int> cfs;                      // lpc slots per device cell: device face id or -1
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;

@@ -14,32 +14,49 @@
 // the serial sweep.  The limiter and the update run redundantly on the
 // lanes of a cell; each lane stores a share of the 5 results.
 //
-// Schedule: the forward sweep runs chunks in forward level order, the
-// backward sweep in the reverse order.  Chunk u of a sweep is handled by
-// wave u mod gridDim of a cooperative (co-resident) grid, each wave
-// walking its chunks in increasing u: the lowest unfinished chunk is always
-// being worked on and its dependencies are finished, so the sweep cannot
+// Schedule: cells are stored by tile (4096 consecutive mesh ids, a compact
+// region in TDTree numbering), then forward level; a chunk is up to 64/LPC
+// cells of one (tile, level) group.  Every forward dependency of a chunk
+// lies in an earlier chunk of the same tile or in an earlier tile, every
+// backward dependency in a later one.  A workgroup of W waves owns a tile:
+// wave w takes the tile's chunks w, w+W, ... in level order (reverse level
+// order for the backward sweep), so dependencies inside the tile are
+// published by waves of the same CU.  Tiles go to workgroups round-robin
+// over a cooperative (co-resident) grid, each workgroup walking its tiles
+// in sweep order and each wave finishing its chunks of one tile before the
+// next.  The first unfinished chunk in sweep order is then always being
+// worked on and its dependencies are finished, so the sweep cannot
 // deadlock.
 //
 // Publication: before a sweep every DQ slot it writes holds a NaN payload
-// (kSentinel) that arithmetic never produces.  A cell writes its 5 final
-// DQ values with device-coherent 64-bit stores; a dependent cell reads
-// them with device-coherent loads and retries while any is the sentinel
-// (each 64-bit store is single-copy atomic, so a non-sentinel value is
-// final).  One lane per wave first waits on the chunk's latest dependency
-// (rep), which keeps the polling traffic to one load per wave.  The forward
-// sweep re-arms the backward array and the backward sweep the forward one.
+// (the sentinel) that arithmetic never produces.  A cell writes its 5 final
+// DQ values with device-coherent stores; a dependent cell reads them with
+// device-coherent loads and retries while any is the sentinel (each store
+// is single-copy atomic, so a non-sentinel value is final).  One lane per
+// wave first waits on the chunk's latest dependency (rep), which keeps the
+// polling traffic to one load per wave.  The forward sweep re-arms the
+// backward array and the backward sweep the forward one.
+//
+// Precision: R is the type of the implicit operator (state snapshot,
+// geometry, Diag, DQ).  With R = double the sweeps are the reference's
+// arithmetic bit for bit.  With R = float (mixed precision) the operator
+// runs in FP32 while the right-hand side (the FP64 residual) and the state
+// update stay FP64, so the iteration is a defect correction whose fixed
+// point is the FP64 solution.
 
-constexpr unsigned long long kSentinel = 0x7FF4D51F5EED0000ULL;
 constexpr int kMaxFaces = 6;  // tetrahedra, pyramids, prisms, hexahedra
 
-__device__ __forceinline__ unsigned long long ld_dev(const double* p) {
-    return __hip_atomic_load(reinterpret_cast<unsigned long long*>(const_cast<double*>(p)), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+constexpr unsigned long long kSentinel = Real<double>::kSent;
+
+template <typename R>
+__device__ __forceinline__ typename Real<R>::bits ld_dev(const R* p) {
+    using B = typename Real<R>::bits;
+    return __hip_atomic_load(reinterpret_cast<B*>(const_cast<R*>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_dev(double* p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __builtin_bit_cast(unsigned long long, v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <typename R>
+__device__ __forceinline__ void st_dev(R* p, R v) {
+    using B = typename Real<R>::bits;
+    __hip_atomic_store(reinterpret_cast<B*>(p), __builtin_bit_cast(B, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Value of lane S of this lane's group of LPC lanes (a quad for LPC = 4,
@@ -55,6 +72,15 @@ __device__ __forceinline__ double cell_bcast(double v) {
         return __shfl(v, (int)(threadIdx.x & ~(unsigned)(LPC - 1)) + S, 64);
     }
 }
+template <int LPC, int S>
+__device__ __forceinline__ float cell_bcast(float v) {
+    if constexpr (LPC == 4) {
+        const int r = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), S * 0x55, 0xf, 0xf, false);
+        return __builtin_bit_cast(float, r);
+    } else {
+        return __shfl(v, (int)(threadIdx.x & ~(unsigned)(LPC - 1)) + S, 64);
+    }
+}
 
 // True when another wave has already given up (checked every 4096 spins),
 // so that one expired wait ends the whole sweep quickly.
@@ -63,41 +89,44 @@ __device__ __forceinline__ bool wait_failed(int* err, unsigned spins) {
     return __shfl(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0, 64) != 0;
 }
 
+template <typename R>
 struct SweepArgs {
     int nC, nchunks;
-    const int2* chunk;   // forward order: first device cell, cell count
-    const int* cfs;      // LPC face slots per device cell (-1 = none)
+    int ntiles;
+    const int* tile_chunk;                  // ntiles+1: first chunk of each tile
+    const int2* chunk;                      // device order: first device cell, cell count
+    const int* cfs;                         // LPC face slots per device cell (-1 = none)
     const int2* fcell;
-    const double4* fgeo;
-    const double* q;     // state read by both sweeps (cell-major, 5 per cell)
-    const double* res;   // forward: initial DQ
-    const double* diag;
-    double* dqf;         // forward DQ, 5 per cell
-    double* dqb;         // backward (final) DQ, 5 per cell incl. ghosts (0)
-    double* qout;        // backward: updated state
-    const int* rep;      // per chunk in run order: dependency published last, -1 if none
-    const int* depmask;  // per cell: forward / backward dependency face slots
-    int cell0;           // cell with mesh id 0 (forward: divide only)
+    const typename Real<R>::vec4* fgeo;     // unit normal + area
+    const R* q;                             // operator state (cell-major, 5 per cell)
+    const double* q64;                      // FP64 state the backward sweep updates
+    const double* res;                      // forward: initial DQ (FP64 residual)
+    const R* diag;
+    R* dqf;                                 // forward DQ, 5 per cell
+    R* dqb;                                 // backward (final) DQ, 5 per cell incl. ghosts (0)
+    double* qout;                           // backward: updated state
+    const int* rep;                         // per chunk: dependency expected to be published last, -1 if none
+    const int* depmask;                     // per cell: forward / backward dependency face slots
+    int cell0;                              // cell with mesh id 0 (forward: divide only)
     int* err;
 };
 
 // Adds the LPC per-face products of this lane's cell in slot order (the
 // cell's ascending face order), taking only the slots in `use`.
-template <bool FWD, int LPC>
-__device__ __forceinline__ void add_cell_terms(const double (&p)[5], unsigned use, double (&D)[5],
-                                               double (&acc)[5]) {
-#define MFS_ADD_SLOT(S)                                                   \
-    if constexpr ((S) < LPC) {                                            \
-        double ps[5];                                                     \
-        _Pragma("unroll") for (int j = 0; j < 5; ++j) ps[j] = cell_bcast<LPC, (S)>(p[j]); \
-        if (use & (1u << (S))) {                                          \
-            _Pragma("unroll") for (int j = 0; j < 5; ++j) {               \
-                if (FWD)                                                  \
-                    D[j] -= ps[j];                                        \
-                else                                                      \
-                    acc[j] += ps[j];                                      \
-            }                                                             \
-        }                                                                 \
+template <bool FWD, int LPC, typename R>
+__device__ __forceinline__ void add_cell_terms(const R (&p)[5], unsigned use, R (&D)[5], R (&acc)[5]) {
+#define MFS_ADD_SLOT(S)                                                                      \
+    if constexpr ((S) < LPC) {                                                               \
+        R ps[5];                                                                             \
+        _Pragma("unroll") for (int j = 0; j < 5; ++j) ps[j] = cell_bcast<LPC, (S)>(p[j]);    \
+        if (use & (1u << (S))) {                                                             \
+            _Pragma("unroll") for (int j = 0; j < 5; ++j) {                                  \
+                if (FWD)                                                                     \
+                    D[j] -= ps[j];                                                           \
+                else                                                                         \
+                    acc[j] += ps[j];                                                         \
+            }                                                                                \
+        }                                                                                    \
     }
     MFS_ADD_SLOT(0)
     MFS_ADD_SLOT(1)
@@ -110,131 +139,163 @@ __device__ __forceinline__ void add_cell_terms(const double (&p)[5], unsigned us
 #undef MFS_ADD_SLOT
 }
 
-template <bool FWD, int LPC>
-__global__ __launch_bounds__(64) void k_sweep(SweepArgs A, FlowConst K) {
-    constexpr int CPC = 64 / LPC;
-    const int lane = (int)threadIdx.x;
+template <typename R>
+__device__ __forceinline__ void load5r(const R* __restrict__ p, R v[5]) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) v[j] = p[j];
+}
+
+// One chunk t (device order) of a sweep.  Returns false when a wait
+// expired (the error word is set).
+template <bool FWD, int LPC, typename R>
+__device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowConst& K, int t) {
+    using B = typename Real<R>::bits;
+    constexpr B kSent = Real<R>::kSent;
+    const int lane = (int)threadIdx.x & 63;
     const int k = lane / LPC;  // cell of the chunk
     const int s = lane % LPC;  // face slot
-    double* const poll = FWD ? A.dqf : A.dqb;
-    // a sweep that follows a failed one in the same stream does not start
-    if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    for (int u = blockIdx.x; u < A.nchunks; u += gridDim.x) {
-        const int t = FWD ? u : A.nchunks - 1 - u;
-        const int2 ch = A.chunk[t];
-        const bool active = k < ch.y;
-        const int c = ch.x + (active ? k : 0);  // idle lanes read the first cell and store nothing
-        const unsigned m = (unsigned)A.depmask[c];
-        unsigned kind = FWD ? (m & 0xffu) : ((m >> 8) & 0xffu);  // slots that take part
-        const unsigned pendm = FWD ? kind : ((m >> 16) & 0xffu);  // slots with a DQ to wait for
-        if (FWD && c == A.cell0) kind = 0;  // forward_lusgs.cpp:803-804: the first cell only divides
-        const bool mine = active && ((kind >> s) & 1u);
-        bool pend = active && ((pendm >> s) & 1u) && mine;
-        // operands that do not depend on the sweep, loaded before waiting
-        double fnx = 0.0, fny = 0.0, fnz = 0.0, far_ = 0.0;
-        double qnb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        double dv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        int nb = 0;
-        if (mine) {
-            const int f = A.cfs[(size_t)c * LPC + s];
-            const int2 cc = A.fcell[f];
-            const double4 g = A.fgeo[f];
-            const bool flip = (cc.y == c);  // normal oriented from c to its neighbour
-            nb = flip ? cc.x : cc.y;
-            fnx = flip ? -g.x : g.x;
-            fny = flip ? -g.y : g.y;
-            fnz = flip ? -g.z : g.z;
-            far_ = g.w;
-            load5(A.q + (size_t)nb * 5, qnb);
-        }
-        double qc[5], D[5];
-        load5(A.q + (size_t)c * 5, qc);
-        const double dg = A.diag[c];
-        load5((FWD ? A.res : A.dqf) + (size_t)c * 5, D);
+    R* const poll = FWD ? A.dqf : A.dqb;
+    const int2 ch = A.chunk[t];
+    const bool active = k < ch.y;
+    const int c = ch.x + (active ? k : 0);  // idle lanes read the first cell and store nothing
+    const unsigned m = (unsigned)A.depmask[c];
+    unsigned kind = FWD ? (m & 0xffu) : ((m >> 8) & 0xffu);  // slots that take part
+    const unsigned pendm = FWD ? kind : ((m >> 16) & 0xffu);  // slots with a DQ to wait for
+    if (FWD && c == A.cell0) kind = 0;  // forward_lusgs.cpp:803-804: the first cell only divides
+    const bool mine = active && ((kind >> s) & 1u);
+    bool pend = active && ((pendm >> s) & 1u) && mine;
+    // operands that do not depend on the sweep, loaded before waiting
+    R fnx = R(0), fny = R(0), fnz = R(0), far_ = R(0);
+    R qnb[5] = {R(0), R(0), R(0), R(0), R(0)};
+    R dv[5] = {R(0), R(0), R(0), R(0), R(0)};
+    int nb = 0;
+    if (mine) {
+        const int f = A.cfs[(size_t)c * LPC + s];
+        const int2 cc = A.fcell[f];
+        const typename Real<R>::vec4 g = A.fgeo[f];
+        const bool flip = (cc.y == c);  // normal oriented from c to its neighbour
+        nb = flip ? cc.x : cc.y;
+        fnx = flip ? -g.x : g.x;
+        fny = flip ? -g.y : g.y;
+        fnz = flip ? -g.z : g.z;
+        far_ = g.w;
+        load5r(A.q + (size_t)nb * 5, qnb);
+    }
+    R qc[5], D[5];
+    load5r(A.q + (size_t)c * 5, qc);
+    const R dg = A.diag[c];
+    if (FWD) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) D[j] = (R)A.res[(size_t)c * 5 + j];
+    } else {
+        load5r(A.dqf + (size_t)c * 5, D);
+    }
 
-        unsigned spins = 0;
-        // one lane waits for the chunk's latest dependency
-        const int rep = A.rep[u];
-        if (rep >= 0) {
-            while (true) {
-                unsigned long long v = 0;
-                if (lane == 0) v = ld_dev(poll + (size_t)rep * 5);
-                v = __shfl(v, 0, 64);
-                if (v != kSentinel) break;
-                if (++spins > kSpinLimit || wait_failed(A.err, spins)) {
-                    if (lane == 0) atomicExch(A.err, 1);
-                    return;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        // every lane reads its face's dependency, retrying the rare unpublished one
+    // one lane waits for the chunk's latest dependency
+    unsigned spins = 0;
+    const int rep = A.rep[t];
+    if (rep >= 0) {
         while (true) {
-            if (pend) {
-                unsigned long long b[5];
-                bool ok = true;
-#pragma unroll
-                for (int j = 0; j < 5; ++j) {
-                    b[j] = ld_dev(poll + (size_t)nb * 5 + j);
-                    ok = ok && (b[j] != kSentinel);
-                }
-                if (ok) {
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) dv[j] = __builtin_bit_cast(double, b[j]);
-                    pend = false;
-                }
-            }
-            if (__ballot(pend) == 0) break;
+            B v = 0;
+            if (lane == 0) v = ld_dev<R>(poll + (size_t)rep * 5);
+            v = __shfl(v, 0, 64);
+            if (v != kSent) break;
             if (++spins > kSpinLimit || wait_failed(A.err, spins)) {
                 if (lane == 0) atomicExch(A.err, 1);
-                return;
+                return false;
             }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(2);
         }
-        // this lane's face term: forward 0.5*A*F (DQ -= term), backward A*F (summed)
-        double p[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (mine) {
-            double fl[5];
-            flux_lusgs<double>(qnb, dv, fnx, fny, fnz, K.gam, K.p_bar, K.lhs_omga, fl);
-            const double tmp = FWD ? 0.5 * far_ : far_;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) p[j] = tmp * fl[j];
-        }
-        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        add_cell_terms<FWD, LPC>(p, kind, D, acc);
-        if (FWD) {
-#pragma unroll
-            for (int j = 0; j < 5; ++j) D[j] /= dg;
-            if (c != A.cell0) dq_limit<double>(qc, D, K.gamm1, K.rho_min, K.rho_max, K.p_min, K.p_max);
-        } else {
-            const double tmp = 2.0 * dg;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) D[j] -= acc[j] / tmp;
-            dq_limit<double>(qc, D, K.gamm1, K.rho_min, K.rho_max, K.p_min, K.p_max);
-        }
-        if (active) {
-            double* const own = FWD ? A.dqf : A.dqb;    // published
-            double* const other = FWD ? A.dqb : A.dqf;  // re-armed for the next sweep
-            const double sent = __builtin_bit_cast(double, kSentinel);
+    }
+    // every lane reads its face's dependency, retrying the rare unpublished one
+    while (true) {
+        if (pend) {
+            B b[5];
+            bool ok = true;
 #pragma unroll
             for (int j = 0; j < 5; ++j) {
-                if (j % LPC == s) {
-                    st_dev(own + (size_t)c * 5 + j, D[j]);
-                    other[(size_t)c * 5 + j] = sent;
-                }
+                b[j] = ld_dev<R>(poll + (size_t)nb * 5 + j);
+                ok = ok && (b[j] != kSent);
+            }
+            if (ok) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) dv[j] = __builtin_bit_cast(R, b[j]);
+                pend = false;
             }
         }
-        if (!FWD) {
-            double qnew[5];
+        if (__ballot(pend) == 0) break;
+        if (++spins > kSpinLimit || wait_failed(A.err, spins)) {
+            if (lane == 0) atomicExch(A.err, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    // this lane's face term: forward 0.5*A*F (DQ -= term), backward A*F (summed)
+    R p[5] = {R(0), R(0), R(0), R(0), R(0)};
+    if (mine) {
+        R fl[5];
+        flux_lusgs<R>(qnb, dv, fnx, fny, fnz, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga, fl);
+        const R tmp = FWD ? R(0.5) * far_ : far_;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) p[j] = tmp * fl[j];
+    }
+    R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
+    add_cell_terms<FWD, LPC, R>(p, kind, D, acc);
+    if (FWD) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) D[j] /= dg;
+        if (c != A.cell0)
+            dq_limit<R>(qc, D, (R)K.gamm1, (R)K.rho_min, (R)K.rho_max, (R)K.p_min, (R)K.p_max);
+    } else {
+        const R tmp = R(2.0) * dg;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) D[j] -= acc[j] / tmp;
+        dq_limit<R>(qc, D, (R)K.gamm1, (R)K.rho_min, (R)K.rho_max, (R)K.p_min, (R)K.p_max);
+    }
+    if (active) {
+        R* const own = FWD ? A.dqf : A.dqb;    // published
+        R* const other = FWD ? A.dqb : A.dqf;  // re-armed for the next sweep
+        const R sent = __builtin_bit_cast(R, kSent);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            if (j % LPC == s) {
+                st_dev<R>(own + (size_t)c * 5 + j, D[j]);
+                other[(size_t)c * 5 + j] = sent;
+            }
+        }
+    }
+    if (!FWD) {
+        double qnew[5], dd[5];
+        if constexpr (sizeof(R) == sizeof(double)) {
 #pragma unroll
             for (int j = 0; j < 5; ++j) qnew[j] = qc[j];
-            update_cell(qnew, D, K);
-            if (active) {
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    if (j % LPC == s) A.qout[(size_t)c * 5 + j] = qnew[j];
-            }
+        } else {
+            load5(A.q64 + (size_t)c * 5, qnew);
         }
-        (void)CPC;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) dd[j] = (double)D[j];
+        update_cell(qnew, dd, K);
+        if (active) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                if (j % LPC == s) A.qout[(size_t)c * 5 + j] = qnew[j];
+        }
+    }
+    return true;
+}
+
+template <bool FWD, int LPC, typename R>
+__global__ __launch_bounds__(kSweepBlock) void k_sweep(SweepArgs<R> A, FlowConst K) {
+    // a sweep that follows a failed one in the same stream does not start
+    if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int W = (int)blockDim.x / 64;
+    const int w = (int)threadIdx.x / 64;
+    for (int kk = blockIdx.x; kk < A.ntiles; kk += gridDim.x) {
+        const int k = FWD ? kk : A.ntiles - 1 - kk;
+        const int c0 = A.tile_chunk[k], c1 = A.tile_chunk[k + 1];
+        for (int j = w; j < c1 - c0; j += W) {
+            const int t = FWD ? c0 + j : c1 - 1 - j;
+            if (!sweep_chunk<FWD, LPC, R>(A, K, t)) return;
+        }
     }
 }

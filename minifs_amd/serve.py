@@ -1,0 +1,215 @@
+"""Serving loop for steady solves of uneven mesh size and flow condition.
+
+Requests arrive over time.  Each one is a box+bump mesh (size, bump height,
+seed, cell kind) and a flow condition, solved to a residual drop.  The
+server overlaps three things:
+
+* host set-up (mesh generation, TDTree ordering, metrics: the C++ host code
+  releases the GIL through ctypes) on a pool of worker threads;
+* admission of set-up solves into one device batch (mfs_batch_*) while the
+  device memory they need fits the budget;
+* batched device iterations of every admitted solve (one launch per kernel
+  for all of them), with convergence checked every `check_every` steps;
+  converged solves leave the batch and free their memory for the next.
+
+Each solve's arithmetic is the single-solve path's, so a result is the same
+alone and in the mix.  `run_sequential` is the baseline: the same requests
+one at a time, each set up and solved alone on the plain FP64 path.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import MFS_FP64, Batch, Grid, Solver, make_params
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    nx: int
+    ny: int
+    nz: int
+    mach: float
+    alpha: float
+    bump_height: float = 0.05
+    seed: int = 20260101
+    cells: str = "tet"
+    # filled by the server
+    t_ready: float = math.nan     # host set-up finished
+    t_admit: float = math.nan     # joined the device batch
+    t_done: float = math.nan      # converged (or gave up)
+    iters: int = 0
+    drop: float = math.nan        # final rho-residual norm / first
+    converged: bool = False
+    n_cells: int = 0
+    rho_sum: float = math.nan     # checksum of the final density
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+def make_requests(n: int, rate: float, cells_lo: float, cells_hi: float, seed: int) -> list[Request]:
+    """Poisson arrivals at `rate` per second; cell counts log-uniform in
+    [cells_lo, cells_hi] as 6 n^3 tetrahedra of a near-cubic box; Mach ~
+    U[0.3, 0.85], AoA ~ U[-3, 5] degrees, bump height ~ U[0.02, 0.08]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        cells = math.exp(rng.uniform(math.log(cells_lo), math.log(cells_hi)))
+        side = (cells / 6.0) ** (1.0 / 3.0)
+        nx = max(2, int(round(side * rng.uniform(0.85, 1.15))))
+        ny = max(2, int(round(side * rng.uniform(0.85, 1.15))))
+        nz = max(2, int(round(cells / 6.0 / (nx * ny))))
+        out.append(Request(i, t, nx, ny, nz, float(rng.uniform(0.3, 0.85)), float(rng.uniform(-3.0, 5.0)),
+                           float(rng.uniform(0.02, 0.08)), int(rng.integers(1, 2**31))))
+    return out
+
+
+def _setup(r: Request, cfl: float):
+    p = make_params(mach=r.mach, alpha=r.alpha, cfl=cfl)
+    g = Grid.box_bump(r.nx, r.ny, r.nz, bump_height=r.bump_height, seed=r.seed, cells=r.cells)
+    g.decouple().preprocess(p)
+    return g, p
+
+
+def _converged(h: np.ndarray, drop: float) -> bool:
+    return len(h) > 0 and h[-1, 0] <= drop * h[0, 0]
+
+
+class Server:
+    def __init__(self, precision: int, hbm_budget: float, drop: float = 1e-8, max_iters: int = 20000,
+                 check_every: int = 20, cfl: float = 10.0, workers: int = 8, bytes_per_cell: float | None = None):
+        self.precision = precision
+        self.budget = hbm_budget
+        self.drop = drop
+        self.max_iters = max_iters
+        self.check_every = check_every
+        self.cfl = cfl
+        self.workers = workers
+        # device bytes per cell, refined from the contexts actually created
+        self.bytes_per_cell = bytes_per_cell or (620.0 if precision == MFS_FP64 else 900.0)
+
+    def run(self, requests: list[Request], progress=None) -> dict:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        batch = Batch(self.precision)
+        pool = cf.ThreadPoolExecutor(self.workers)
+        start = time.perf_counter()
+        now = lambda: time.perf_counter() - start  # noqa: E731
+        nxt = 0                     # next request not yet submitted for set-up
+        building: dict[cf.Future, Request] = {}
+        ready: list[tuple[Request, Grid, object]] = []
+        live: dict[int, tuple[Request, object, Grid]] = {}   # slot -> (request, member, grid)
+        used = 0.0
+        done = []
+        peak_live = 0
+        while len(done) < len(reqs):
+            t = now()
+            while nxt < len(reqs) and reqs[nxt].arrival <= t:
+                building[pool.submit(_setup, reqs[nxt], self.cfl)] = reqs[nxt]
+                nxt += 1
+            for fut in [f for f in building if f.done()]:
+                r = building.pop(fut)
+                g, p = fut.result()
+                r.t_ready = now()
+                ready.append((r, g, p))
+            # admission in arrival order while the memory budget allows
+            while ready:
+                r, g, p = ready[0]
+                nc = g.info["nTCell"]
+                need = nc * self.bytes_per_cell
+                if live and used + need > self.budget:
+                    break
+                ready.pop(0)
+                m = batch.add(g, p)
+                r.t_admit = now()
+                r.n_cells = nc
+                dev_bytes = m.info()["device_bytes"]
+                self.bytes_per_cell = max(self.bytes_per_cell, dev_bytes / nc)
+                used += dev_bytes
+                live[m.slot] = (r, m, g)
+            peak_live = max(peak_live, len(live))
+            if live:
+                batch.step(self.check_every)
+                batch.sync()
+                for slot, (r, m, g) in list(live.items()):
+                    h = m.history()
+                    if _converged(h, self.drop) or len(h) >= self.max_iters:
+                        r.t_done = now()
+                        r.iters = len(h)
+                        r.drop = float(h[-1, 0] / h[0, 0])
+                        r.converged = _converged(h, self.drop)
+                        r.rho_sum = float(np.sum(m.field("rho")))
+                        used -= m.info()["device_bytes"]
+                        batch.remove(m)
+                        g.close()
+                        del live[slot]
+                        done.append(r)
+                        if progress:
+                            progress(r)
+            else:
+                # nothing on the device: wait for the next set-up or arrival
+                if building:
+                    cf.wait(list(building), timeout=0.05, return_when=cf.FIRST_COMPLETED)
+                elif nxt < len(reqs):
+                    time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+        wall = now()
+        pool.shutdown()
+        batch.close()
+        return summarize(done, wall, peak_live)
+
+
+def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int = 20000, check_every: int = 20,
+                   cfl: float = 10.0, precision: int = MFS_FP64, progress=None) -> dict:
+    """The baseline: requests served one at a time in arrival order, each set
+    up and solved alone."""
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    start = time.perf_counter()
+    now = lambda: time.perf_counter() - start  # noqa: E731
+    done = []
+    for r in reqs:
+        wait = r.arrival - now()
+        if wait > 0:
+            time.sleep(wait)
+        g, p = _setup(r, cfl)
+        r.t_ready = r.t_admit = now()
+        s = Solver(g, p, precision=precision)
+        r.n_cells = g.info["nTCell"]
+        while True:
+            s.step(check_every)
+            h = s.history()
+            if _converged(h, drop) or len(h) >= max_iters:
+                break
+        r.t_done = now()
+        r.iters = len(h)
+        r.drop = float(h[-1, 0] / h[0, 0])
+        r.converged = _converged(h, drop)
+        r.rho_sum = float(np.sum(s.field("rho")))
+        s.close()
+        g.close()
+        done.append(r)
+        if progress:
+            progress(r)
+    return summarize(done, now(), 1)
+
+
+def summarize(done: list[Request], wall: float, peak_live: int) -> dict:
+    lat = np.array([r.latency for r in done])
+    return {
+        "solves": len(done),
+        "converged": int(sum(r.converged for r in done)),
+        "wall_s": round(wall, 3),
+        "solves_per_hour": round(len(done) / wall * 3600.0, 1) if wall > 0 else None,
+        "p50_s": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+        "p99_s": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+        "cell_iters": int(sum(r.n_cells * r.iters for r in done)),
+        "peak_concurrent": peak_live,
+    }

@@ -1,0 +1,79 @@
+"""GPU tests of the batched multi-solve (mfs_batch_*): every solve of a
+batch gives the same bits as the same solve run alone, also when solves
+join and leave the batch between steps."""
+import numpy as np
+import pytest
+
+import minifs_amd as mfs
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "u", "v", "w", "p", "res", "dt", "dq", "dqdx", "dqdy", "dqdz")
+
+
+def _case(n, mach, alpha, seed, cells="tet"):
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl=10.0)
+    g = mfs.Grid.box_bump(n, seed=seed, cells=cells).decouple().preprocess(p)
+    return g, p
+
+
+def _alone(g, p, iters, precision):
+    s = mfs.Solver(g, p, precision=precision)
+    s.step(iters)
+    out = {nm: s.field(nm) for nm in FIELDS}
+    h = s.history()
+    s.close()
+    return out, h
+
+
+def _same(member, ref, href):
+    for nm in FIELDS:
+        np.testing.assert_array_equal(member.field(nm), ref[nm], err_msg=nm)
+    np.testing.assert_array_equal(member.history(), href)
+
+
+@pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
+def test_batch_members_equal_standalone(precision):
+    print(f"[batch] precision={precision}", flush=True)
+    cases = [_case(6, 0.5, 0.0, 1), _case(9, 0.8, 1.25, 2), _case(4, 0.3, -3.0, 3)]
+    b = mfs.Batch(precision)
+    members = [b.add(g, p) for g, p in cases]
+    assert len(b) == 3
+    b.step(4)
+    b.step(3)
+    for m, (g, p) in zip(members, cases):
+        ref, href = _alone(g, p, 7, precision)
+        _same(m, ref, href)
+    b.close()
+
+
+def test_batch_join_and_leave():
+    a, c, d = _case(7, 0.6, 1.0, 11), _case(5, 0.7, 2.0, 12), _case(8, 0.45, -1.0, 13)
+    b = mfs.Batch(mfs.MFS_FP64)
+    ma = b.add(*a)
+    mc = b.add(*c)
+    b.step(3)
+    md = b.add(*d)      # joins after 3 steps
+    b.step(2)
+    b.remove(mc)        # leaves after 5 steps
+    b.step(4)
+    for m, case, iters in ((ma, a, 9), (md, d, 6)):
+        ref, href = _alone(*case, iters, mfs.MFS_FP64)
+        _same(m, ref, href)
+    assert len(b) == 2
+    b.close()
+
+
+def test_batch_polyhedral_family():
+    cases = [_case(4, 0.5, 0.0, 21, "hex"), _case(3, 0.8, 1.0, 22, "prism")]
+    b = mfs.Batch(mfs.MFS_FP64)
+    members = [b.add(g, p) for g, p in cases]
+    b.step(5)
+    for m, (g, p) in zip(members, cases):
+        ref, href = _alone(g, p, 5, mfs.MFS_FP64)
+        _same(m, ref, href)
+    # a tetrahedral mesh cannot join a batch of 6-face cells
+    g, p = _case(3, 0.5, 0.0, 23)
+    with pytest.raises(mfs.MfsError):
+        b.add(g, p)
+    b.close()
