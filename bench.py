@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the MiniFS solver path on one MI355X.
 
-Workload (default): the box + Gaussian-bump mesh at 100x100x100 hexes x 6
+Default workload: the box + Gaussian-bump mesh at 100x100x100 hexes x 6
 tetrahedra (6.0M cells, 12.06M faces), Mach 0.5, AoA 0, CFL 10, plain FP64,
 TDTree-ordered.  One step is one iteration of the reference driver loop
 (src/main.cpp:201-291): residual (Roe flux), local time step, LU-SGS
@@ -11,7 +11,18 @@ The mesh and state are resident in HBM before the timed region.
 Prints one JSON line.  `roofline` describes the kernel with the largest
 share of the timed region, timed with HIP events on the solver stream;
 `cpu_baseline` times the reference's own serial build (oracle/_ref) on a
-bounded sample of the same workload on this host.
+bounded sample of the same workload on this host; `mixed` repeats the
+timed region with the mixed-precision context (FP32 implicit operator,
+FP64 residual and update) on the same mesh.
+
+Other modes (one JSON line each):
+  --mode converge  both precisions run to a residual drop on the same mesh
+                   and flow (default Mach 0.8, AoA 1.25); errors of the
+                   mixed run against the FP64 run
+  --mode batch     many independent solves of random size and flow stepped
+                   together (mfs_batch_*), aggregate throughput
+  --mode serve     Poisson arrivals of solves of random size, served by the
+                   batching server against one-at-a-time plain FP64 solves
 """
 from __future__ import annotations
 
@@ -40,9 +51,9 @@ KERNEL_BYTES = {
     # flux and time step in one pass (mfs_step): face tables once, q + vol read,
     # res + dt + Diag written, ghost q
     "flux_ts": (56, 104, 40, 0),
-    # forward sweep: faces, q, res, Diag read, DQ write, ghost q (SURVEY K4)
+    # forward sweep: faces, q, res, Diag read, DQ write, ghost q
     "fwd_sweep": (40, 144, 40, 0),
-    # backward sweep + update: as the forward sweep plus q write (K5 + K6)
+    # backward sweep + update: as the forward sweep plus the state update
     "bwd_sweep": (40, 264, 40, 0),
     # node values: 4 weighted entries per tetrahedron (index + weight), q, node write
     "node": (0, 88, 76, 52),
@@ -138,32 +149,27 @@ def copy_calibration(mib: int, reps: int) -> int:
     return 0
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=100, help="hexes per axis (cells = 6 n^3)")
-    ap.add_argument("--mach", type=float, default=0.5)
-    ap.add_argument("--alpha", type=float, default=0.0)
-    ap.add_argument("--cfl", type=float, default=10.0)
-    ap.add_argument("--seed", type=int, default=20260101)
-    ap.add_argument("--cpu-n", type=int, default=40, help="mesh of the CPU baseline sample (0 = skip)")
-    ap.add_argument("--cpu-iters", type=int, default=20)
-    ap.add_argument("--cpu-port-iters", type=int, default=60)
-    ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain copy of MIB MiB (for scaling rocprofv3 byte counters)")
-    args = ap.parse_args()
-    if args.copy_calibration:
-        return copy_calibration(args.copy_calibration, 5)
+def timed_steps(s, steps: int, warmup: int, kernel_timing: bool):
+    """Warm-up steps, then `steps` timed steps bracketed by synchronisation;
+    returns wall seconds and the per-kernel event times."""
+    s.step(warmup)
+    s.sync()
+    s.enable_timing(kernel_timing)
+    s.reset_timing()
+    t1 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    elapsed = time.perf_counter() - t1
+    kern = {}
+    for k in KERNEL_BYTES:
+        ms, cnt = s.timing(k)
+        if cnt:
+            kern[k] = (ms, cnt)
+    s.enable_timing(False)
+    return elapsed, kern
 
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
-              file=sys.stderr)
-        return 2
-    rank = int(os.environ.get("RANK", "0"))
 
+def mode_fp64(args) -> dict:
     import minifs_amd as mfs
 
     t0 = time.time()
@@ -173,24 +179,9 @@ def main() -> int:
     s = mfs.Solver(g, p)
     setup_s = time.time() - t0
     nC, nF, nB, nN = info["nTCell"], info["nTFace"], info["nBFace"], info["nTNode"]
-
-    s.step(args.warmup)
-    s.sync()
-    s.enable_timing(not args.no_kernel_timing)
-    s.reset_timing()
-    t1 = time.perf_counter()
-    s.step(args.steps)
-    s.sync()
-    t2 = time.perf_counter()
-    elapsed = t2 - t1
+    elapsed, kern = timed_steps(s, args.steps, args.warmup, not args.no_kernel_timing)
     ms_per_step = elapsed / args.steps * 1e3
     value = nC * args.steps / elapsed / 1e6
-
-    kern = {}
-    for k in KERNEL_BYTES:
-        ms, cnt = s.timing(k)
-        if cnt:
-            kern[k] = (ms, cnt)
     hist = s.history()
     roof = None
     if kern:
@@ -208,7 +199,19 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "alg_bytes_per_launch": int(b), "avg_launch_ms": round(ms / cnt, 4)}
     iter_b = alg_bytes(ITER_BYTES, nF, nC, nB, nN)
-    line = {
+    info_ctx = s.info()
+    s.close()
+    mixed = None
+    if not args.no_mixed:
+        sm = mfs.Solver(g, p, precision=mfs.MFS_MIXED)
+        el_m, kern_m = timed_steps(sm, args.steps, args.warmup, not args.no_kernel_timing)
+        mixed = {"value": round(nC * args.steps / el_m / 1e6, 3), "unit": "M cell-iters/s",
+                 "ms_per_step": round(el_m / args.steps * 1e3, 4), "speedup_vs_fp64": round(elapsed / el_m, 3),
+                 "dtype": "f32 operator / f64 residual+update",
+                 "kernel_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in kern_m.items()}}
+        sm.close()
+    g.close()
+    return {
         "metric": METRIC,
         "value": round(value, 3),
         "unit": "M cell-iters/s",
@@ -223,29 +226,215 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"plain FP64 LU-SGS iteration, box+bump {args.n}^3x6 tets", "cells": nC,
                    "faces": nF, "bfaces": nB, "nodes": nN, "mach": args.mach, "alpha_deg": args.alpha,
-                   "cfl": args.cfl, "seed": args.seed, "ordering": "TDTree", "sweep_levels": s.info()["sweep_levels"]},
+                   "cfl": args.cfl, "seed": args.seed, "ordering": "TDTree", "sweep_levels": info_ctx["sweep_levels"]},
         "roofline": roof,
         "iteration_roofline": {"alg_bytes_per_iter": int(iter_b),
                                "achieved_GBs": round(iter_b / (ms_per_step / 1e3) / 1e9, 2),
                                "frac": round(iter_b / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)},
         "kernel_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
+        "mixed": mixed,
         "setup_s": round(setup_s, 2),
         "last_residual": [float(x) for x in hist[-1][:5]] if len(hist) else None,
         "cpu_baseline": None,
         "cpu_port_threaded": None,
     }
-    s.close()
+
+
+def mode_converge(args) -> dict:
+    """Both precisions to a residual drop on one mesh: iterations, time, and
+    the mixed run's state and wall-force errors against the FP64 run."""
+    import numpy as np
+
+    import minifs_amd as mfs
+
+    p = mfs.make_params(mach=args.mach, alpha=args.alpha, cfl=args.cfl)
+    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple().preprocess(p)
+    nC = g.info["nTCell"]
+    runs = {}
+    for prec, name in ((mfs.MFS_FP64, "fp64"), (mfs.MFS_MIXED, "mixed")):
+        s = mfs.Solver(g, p, precision=prec)
+        t1 = time.perf_counter()
+        done = 0
+        while done < args.max_iters:
+            k = min(args.check_every, args.max_iters - done)
+            s.step(k)
+            done += k
+            h = s.history()
+            if h[-1, 0] <= args.drop * h[0, 0]:
+                break
+        s.sync()
+        el = time.perf_counter() - t1
+        st = {nm: s.field(nm) for nm in ("rho", "u", "v", "w", "p")}
+        F, lift = mfs.wall_force(g, st["p"], p)
+        runs[name] = {"iters": len(h), "seconds": round(el, 3), "drop": float(h[-1, 0] / h[0, 0]),
+                      "M_cell_iters_per_s": round(nC * len(h) / el / 1e6, 3), "lift": lift, "state": st}
+        print(f"[converge] {name}: {len(h)} iterations, {el:.1f} s, drop {h[-1, 0] / h[0, 0]:.3e}", file=sys.stderr,
+              flush=True)
+        s.close()
+    err = {}
+    for nm in ("rho", "u", "v", "w", "p"):
+        a, b = runs["fp64"]["state"][nm], runs["mixed"]["state"][nm]
+        scale = (p.p_bar if nm == "p" else float(np.max(np.abs(a))))
+        err[nm] = {"l2_rel": float(np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300)),
+                   "linf_rel": float(np.max(np.abs(a - b)) / scale)}
+    for r in runs.values():
+        r.pop("state")
     g.close()
-    if rank == 0 and args.cpu_n > 0:
-        try:
-            line["cpu_baseline"] = cpu_sample(args.cpu_n, args.cpu_iters, args.mach, args.alpha, args.cfl, args.seed)
-        except Exception as e:  # reported, not fatal: the GPU number stands on its own
-            line["cpu_baseline"] = {"error": str(e)}
-        try:
-            line["cpu_port_threaded"] = cpu_port_sample(args.cpu_n, args.cpu_port_iters, args.mach, args.alpha,
-                                                        args.cfl, args.seed)
-        except Exception as e:
-            line["cpu_port_threaded"] = {"error": str(e)}
+    return {"metric": "converged mixed vs FP64 solve", "mode": "converge",
+            "config": {"workload": f"box+bump {args.n}^3x6 tets", "cells": nC, "mach": args.mach,
+                       "alpha_deg": args.alpha, "cfl": args.cfl, "drop": args.drop},
+            "fp64": runs["fp64"], "mixed": runs["mixed"],
+            "speedup_time_to_converge": round(runs["fp64"]["seconds"] / runs["mixed"]["seconds"], 3),
+            "state_error": err,
+            "lift_rel_error": abs(runs["mixed"]["lift"] - runs["fp64"]["lift"]) / abs(runs["fp64"]["lift"])}
+
+
+def mode_batch(args) -> dict:
+    """Independent solves with nx,ny,nz ~ U[lo,hi], bump ~ U[.02,.08], Mach ~
+    U[.3,.85], AoA ~ U[-3,5] (seeded), stepped together."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    import minifs_amd as mfs
+
+    rng = np.random.default_rng(args.batch_seed)
+    specs = [(int(rng.integers(args.batch_lo, args.batch_hi + 1)), int(rng.integers(args.batch_lo, args.batch_hi + 1)),
+              int(rng.integers(args.batch_lo, args.batch_hi + 1)), float(rng.uniform(0.02, 0.08)),
+              float(rng.uniform(0.3, 0.85)), float(rng.uniform(-3.0, 5.0)), int(rng.integers(1, 2**31)))
+             for _ in range(args.solves)]
+    prec = mfs.MFS_MIXED if args.precision == "mixed" else mfs.MFS_FP64
+
+    def build(sp):
+        nx, ny, nz, bh, mach, alpha, seed = sp
+        p = mfs.make_params(mach=mach, alpha=alpha, cfl=args.cfl)
+        return mfs.Grid.box_bump(nx, ny, nz, bump_height=bh, seed=seed).decouple().preprocess(p), p
+
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(min(16, os.cpu_count() or 4)) as pool:
+        built = list(pool.map(build, specs))
+    setup_s = time.time() - t0
+    b = mfs.Batch(prec)
+    members = [b.add(g, p) for g, p in built]
+    cells = sum(g.info["nTCell"] for g, _ in built)
+    b.step(args.warmup)
+    b.sync()
+    t1 = time.perf_counter()
+    b.step(args.steps)
+    b.sync()
+    el = time.perf_counter() - t1
+    out = {"metric": "aggregate M cell-iters/s of a batch", "mode": "batch", "value": round(cells * args.steps / el / 1e6, 3),
+           "unit": "M cell-iters/s", "ms_per_step": round(el / args.steps * 1e3, 4), "steps": args.steps,
+           "config": {"solves": args.solves, "cells_total": cells, "hex_per_axis": [args.batch_lo, args.batch_hi],
+                      "seed": args.batch_seed, "precision": args.precision}, "setup_s": round(setup_s, 2)}
+    # the same solves one at a time (same precision) for the batching gain
+    if not args.no_sequential:
+        t2 = time.perf_counter()
+        for g, p in built:
+            s = mfs.Solver(g, p, precision=prec)
+            s.step(args.steps)
+            s.sync()
+            s.close()
+        out["sequential_value"] = round(cells * args.steps / (time.perf_counter() - t2) / 1e6, 3)
+    b.close()
+    for g, _ in built:
+        g.close()
+    return out
+
+
+def mode_serve(args) -> dict:
+    import minifs_amd as mfs
+    from minifs_amd.serve import Server, make_requests, run_sequential
+
+    prec = mfs.MFS_MIXED if args.precision == "mixed" else mfs.MFS_FP64
+    reqs = make_requests(args.requests, args.rate, args.cells_lo, args.cells_hi, args.serve_seed)
+    srv = Server(prec, hbm_budget=args.hbm_budget, drop=args.serve_drop, max_iters=args.max_iters,
+                 check_every=args.check_every, workers=min(16, os.cpu_count() or 4))
+    served = srv.run(reqs, progress=lambda r: print(f"[serve] request {r.rid} {r.n_cells} cells {r.iters} it "
+                                                    f"{r.latency:.2f} s", file=sys.stderr, flush=True))
+    base = None
+    if not args.no_sequential:
+        again = make_requests(args.requests, args.rate, args.cells_lo, args.cells_hi, args.serve_seed)
+        base = run_sequential(again, drop=args.serve_drop, max_iters=args.max_iters, check_every=args.check_every)
+        same = all(a.rho_sum == b.rho_sum and a.iters == b.iters
+                   for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(again, key=lambda r: r.rid))) \
+            if prec == mfs.MFS_FP64 else None
+        base["results_equal_to_served"] = same
+    return {"metric": "served solves/hr + p50/p99 time-to-converge", "mode": "serve", "served": served,
+            "sequential_fp64": base,
+            "solves_per_hour_gain": round(served["solves_per_hour"] / base["solves_per_hour"], 3) if base else None,
+            "config": {"requests": args.requests, "rate_per_s": args.rate, "cells": [args.cells_lo, args.cells_hi],
+                       "drop": args.serve_drop, "precision": args.precision, "hbm_budget": args.hbm_budget,
+                       "seed": args.serve_seed}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--mode", choices=["fp64", "converge", "batch", "serve"], default="fp64")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=100, help="hexes per axis (cells = 6 n^3)")
+    ap.add_argument("--mach", type=float, default=None, help="default 0.5 (0.8 for --mode converge)")
+    ap.add_argument("--alpha", type=float, default=None, help="default 0 (1.25 for --mode converge)")
+    ap.add_argument("--cfl", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--cpu-n", type=int, default=40, help="mesh of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-port-iters", type=int, default=60)
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision repeat of the timed region")
+    ap.add_argument("--drop", type=float, default=1e-10, help="converge mode: residual drop")
+    ap.add_argument("--max-iters", type=int, default=20000)
+    ap.add_argument("--check-every", type=int, default=50)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed", help="batch and serve modes")
+    ap.add_argument("--solves", type=int, default=64)
+    ap.add_argument("--batch-lo", type=int, default=15)
+    ap.add_argument("--batch-hi", type=int, default=60)
+    ap.add_argument("--batch-seed", type=int, default=4001)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--cells-lo", type=float, default=20000)
+    ap.add_argument("--cells-hi", type=float, default=3000000)
+    ap.add_argument("--serve-drop", type=float, default=1e-8)
+    ap.add_argument("--serve-seed", type=int, default=5001)
+    ap.add_argument("--hbm-budget", type=float, default=200e9)
+    ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (for scaling rocprofv3 byte counters)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration, 5)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+    conv = args.mode == "converge"
+    if args.mach is None:
+        args.mach = 0.8 if conv else 0.5
+    if args.alpha is None:
+        args.alpha = 1.25 if conv else 0.0
+    rank = int(os.environ.get("RANK", "0"))
+
+    if args.mode == "converge":
+        line = mode_converge(args)
+    elif args.mode == "batch":
+        line = mode_batch(args)
+    elif args.mode == "serve":
+        line = mode_serve(args)
+    else:
+        line = mode_fp64(args)
+        if rank == 0 and args.cpu_n > 0:
+            try:
+                line["cpu_baseline"] = cpu_sample(args.cpu_n, args.cpu_iters, args.mach, args.alpha, args.cfl,
+                                                  args.seed)
+            except Exception as e:  # reported, not fatal: the GPU number stands on its own
+                line["cpu_baseline"] = {"error": str(e)}
+            try:
+                line["cpu_port_threaded"] = cpu_port_sample(args.cpu_n, args.cpu_port_iters, args.mach, args.alpha,
+                                                            args.cfl, args.seed)
+            except Exception as e:
+                line["cpu_port_threaded"] = {"error": str(e)}
     if rank == 0:
         print(json.dumps(line), flush=True)
     return 0

@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib
 from ._lib import FIELD_IDS, MFS_FP64, MFS_MIXED, MfsError, check, lib
 
-__all__ = ["Grid", "Solver", "Batch", "make_params", "MFS_FP64", "MFS_MIXED", "MfsError", "lib"]
+__all__ = ["Grid", "Solver", "Batch", "make_params", "wall_force", "MFS_FP64", "MFS_MIXED", "MfsError", "lib"]
 
 BC_WALL, BC_SYMM, BC_FAR_FIELD, BC_FAR_FIELD_MOD, BC_INTERFACE = 3, 4, 6, 61, 10
 
@@ -108,6 +108,21 @@ class Grid:
             self.close()
         except Exception:
             pass
+
+
+def wall_force(grid: Grid, p_field: np.ndarray, params: _lib.MfsParams) -> tuple[np.ndarray, float]:
+    """Pressure force on the WALL patch of a preprocessed grid,
+    F = sum over wall faces of (p_c1 + p_bar) n A (p is the gauge pressure
+    of the reference's state, reference layout), and the lift-like component
+    L = -F_x sin(alpha) + F_z cos(alpha).  Not part of the reference's loop:
+    the figure the mixed-precision results are compared on."""
+    f2c = grid.array("f2c").reshape(-1, 2)
+    wall = np.flatnonzero(grid.array("bctype") == 3)  # WALL (src/Base/constant.h)
+    pp = np.asarray(p_field)[f2c[wall, 0]] + params.p_bar
+    area = grid.array("area")[wall]
+    F = np.array([np.sum(pp * grid.array(nm)[wall] * area) for nm in ("xfn", "yfn", "zfn")])
+    a = np.deg2rad(params.alpha)
+    return F, float(-F[0] * np.sin(a) + F[2] * np.cos(a))
 
 
 class Solver:

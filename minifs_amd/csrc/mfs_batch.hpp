@@ -36,6 +36,8 @@ struct BatchSolve {
     const int *fno, *fn, *btype;
     const typename Real<R>::vec4* fgeo_r;
     R *gx, *gy, *gz;
+    const int4* cnode;
+    const int* cnperm;
     // sweeps
     SweepArgs<R> sw;
 };
@@ -101,5 +103,5 @@ __global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restri
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
     grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
-                       S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
+                       S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz, S.cnode, S.cnperm);
 }

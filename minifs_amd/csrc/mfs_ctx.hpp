@@ -63,6 +63,8 @@ struct mfs_ctx {
     int *fno = nullptr, *fn = nullptr, *btype = nullptr;
     // cells
     int *cfo = nullptr, *cf = nullptr;
+    int4* cnode = nullptr;   // tetrahedral meshes: 4 nodes per cell
+    int* cnperm = nullptr;   // and each face's node positions
     double* vol = nullptr;
     // node gather
     int* nlo = nullptr;

@@ -322,7 +322,7 @@ bool launch_gradient_t(mfs_ctx* ctx) {
         Timed t(ctx, "gradient");
         hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
                            ctx->cfo, ctx->cf, ctx->fcell, fgeo, ctx->fno, ctx->fn, ctx->btype, q, qn, ctx->vol, gx, gy,
-                           gz);
+                           gz, ctx->cnode, ctx->cnperm);
         if (!check_launch(ctx, "k_grad")) return false;
     }
     return true;
@@ -451,6 +451,12 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ok = ok && dalloc(ctx, ctx->fcell, L.nF, "fcell") && dalloc(ctx, ctx->fgeo, L.nF, "fgeo") &&
          dalloc(ctx, ctx->fdn, L.nF, "fdn") && dalloc(ctx, ctx->fno, L.nF + 1, "fno") &&
          dalloc(ctx, ctx->fn, L.fn.size(), "fn") && dalloc(ctx, ctx->btype, L.nB, "btype");
+    if (ctx->maxf == 4 && L.cnode.empty()) {
+        set_error("mfs_ctx_create: tetrahedral mesh without cell node lists");
+        ok = false;
+    }
+    if (!L.cnode.empty())
+        ok = ok && dalloc(ctx, ctx->cnode, (size_t)L.nC, "cnode") && dalloc(ctx, ctx->cnperm, (size_t)L.nC, "cnperm");
     ok = ok && dalloc(ctx, ctx->cfo, L.nC + 1, "cfo") && dalloc(ctx, ctx->cf, L.cf.size(), "cf") &&
          dalloc(ctx, ctx->vol, L.nC, "vol");
     ok = ok && dalloc(ctx, ctx->nlo, L.nN + 1, "nlo") && dalloc(ctx, ctx->nlab, L.nla.size(), "nlab") &&
@@ -511,6 +517,9 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
              upload(ctx, ctx->chunk, reinterpret_cast<const int2*>(L.chunk.data()), (size_t)ctx->nchunk, "upload chunk") &&
              upload(ctx, ctx->cfs, L.cfs.data(), L.cfs.size(), "upload cfs") &&
+             (L.cnode.empty() ||
+              (upload(ctx, ctx->cnode, reinterpret_cast<const int4*>(L.cnode.data()), (size_t)L.nC, "upload cnode") &&
+               upload(ctx, ctx->cnperm, L.cnperm.data(), (size_t)L.nC, "upload cnperm"))) &&
              upload(ctx, ctx->tile_chunk, L.tile_chunk.data(), L.tile_chunk.size(), "upload tile_chunk") &&
              upload(ctx, ctx->rep_f, L.rep_f.data(), L.rep_f.size(), "upload rep_f") &&
              upload(ctx, ctx->rep_b, L.rep_b.data(), L.rep_b.size(), "upload rep_b") &&
@@ -1008,6 +1017,8 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     E.fno = c->fno;
     E.fn = c->fn;
     E.btype = c->btype;
+    E.cnode = c->cnode;
+    E.cnperm = c->cnperm;
     auto& A = E.sw;
     A.nC = c->nC;
     A.nchunks = c->nchunk;

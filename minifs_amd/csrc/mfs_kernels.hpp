@@ -524,6 +524,11 @@ __global__ void k_node_symm(int nsym, const int* __restrict__ sym_node, const do
 // 1889-1897): face values from the node values (interior faces, SYMM and
 // INTERFACE boundary faces) or the face-centre average of cell and ghost;
 // outputs in the reference's per-variable blocks of nC+nB.
+template <typename R>
+__device__ __forceinline__ R pick4(const R (&nv)[4][5], int i, int j) {
+    return i == 0 ? nv[0][j] : (i == 1 ? nv[1][j] : (i == 2 ? nv[2][j] : nv[3][j]));
+}
+
 template <int MAXF, typename R>
 __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __restrict__ cfo,
                                           const int* __restrict__ cf, const int2* __restrict__ fcell,
@@ -531,7 +536,8 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
                                           const int* __restrict__ fno, const int* __restrict__ fn,
                                           const int* __restrict__ btype, const double* __restrict__ q,
                                           const R* __restrict__ qn, const double* __restrict__ vol,
-                                          R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz) {
+                                          R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz,
+                                          const int4* __restrict__ cnode, const int* __restrict__ cnperm) {
     if (c >= nC) return;
     R ax[5] = {R(0), R(0), R(0), R(0), R(0)}, ay[5] = {R(0), R(0), R(0), R(0), R(0)},
       az[5] = {R(0), R(0), R(0), R(0), R(0)};
@@ -539,16 +545,23 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
     const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
     int2 cc[MAXF];
     typename Real<R>::vec4 g[MAXF];
-    // tetrahedral meshes: every face is a triangle and its nodes sit at 3f
-    int tn[MAXF == 4 ? 12 : 1];
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
         cc[s] = fcell[fs[s]];
         g[s] = fgeo[fs[s]];
-        if constexpr (MAXF == 4) {
+    }
+    // tetrahedra: the cell's 4 node values, gathered once; face s takes
+    // three of them in its own node order (positions in cnperm)
+    R nv[MAXF == 4 ? 4 : 1][5];
+    int perm = 0;
+    if constexpr (MAXF == 4) {
+        const int4 cn = cnode[c];
+        perm = cnperm[c];
+        const int ids[4] = {cn.x, cn.y, cn.z, cn.w};
 #pragma unroll
-            for (int i = 0; i < 3; ++i) tn[3 * s + i] = fn[3 * fs[s] + i];
-        }
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) nv[a][j] = qn[(size_t)ids[a] * 5 + j];
     }
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
@@ -564,18 +577,13 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
             R sum[5] = {R(0), R(0), R(0), R(0), R(0)};
             R cnt;
             if constexpr (MAXF == 4) {
-                R v0[5], v1[5], v2[5];
+                const int p = perm >> (6 * s);
+                const int i0 = p & 3, i1 = (p >> 2) & 3, i2 = (p >> 4) & 3;
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
-                    v0[j] = qn[(size_t)tn[3 * s] * 5 + j];
-                    v1[j] = qn[(size_t)tn[3 * s + 1] * 5 + j];
-                    v2[j] = qn[(size_t)tn[3 * s + 2] * 5 + j];
-                }
-#pragma unroll
-                for (int j = 0; j < 5; ++j) {
-                    sum[j] += v0[j];
-                    sum[j] += v1[j];
-                    sum[j] += v2[j];
+                    sum[j] += pick4<R>(nv, i0, j);
+                    sum[j] += pick4<R>(nv, i1, j);
+                    sum[j] += pick4<R>(nv, i2, j);
                 }
                 cnt = R(3.0);
             } else {
@@ -625,7 +633,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __res
                                                  const int* __restrict__ fno, const int* __restrict__ fn,
                                                  const int* __restrict__ btype, const double* __restrict__ q,
                                                  const R* __restrict__ qn, const double* __restrict__ vol,
-                                                 R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz) {
+                                                 R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz,
+                                                 const int4* __restrict__ cnode, const int* __restrict__ cnperm) {
     grad_body<MAXF, R>(xcd_block() * kBlock + (int)threadIdx.x, nC, n, cfo, cf, fcell, fgeo, fno, fn, btype, q, qn,
-                       vol, gx, gy, gz);
+                       vol, gx, gy, gz, cnode, cnperm);
 }

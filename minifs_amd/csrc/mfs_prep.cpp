@@ -432,6 +432,47 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
             for (int k = L.cfo[d]; k < L.cfo[d + 1]; ++k) L.cfs[(size_t)d * L.lpc + (k - L.cfo[d])] = L.cf[k];
     }
 
+    // ---- tetrahedral meshes: cell node lists for the gradient's face means
+    L.cnode.clear();
+    L.cnperm.clear();
+    if (L.max_faces == 4) {
+        bool tet = true;
+        L.cnode.assign(4 * (size_t)nC, -1);
+        L.cnperm.assign(nC, 0);
+        for (int d = 0; d < nC && tet; ++d) {
+            int* cn = &L.cnode[4 * (size_t)d];
+            int nn = 0, perm = 0;
+            for (int k = L.cfo[d]; k < L.cfo[d + 1] && tet; ++k) {
+                const int f = L.cf[k];
+                if (L.fno[f + 1] - L.fno[f] != 3) {
+                    tet = false;
+                    break;
+                }
+                for (int i = 0; i < 3; ++i) {
+                    const int node = L.fn[L.fno[f] + i];
+                    int pos = -1;
+                    for (int a = 0; a < nn; ++a)
+                        if (cn[a] == node) pos = a;
+                    if (pos < 0) {
+                        if (nn == 4) {
+                            tet = false;
+                            break;
+                        }
+                        pos = nn;
+                        cn[nn++] = node;
+                    }
+                    perm |= pos << (6 * (k - L.cfo[d]) + 2 * i);
+                }
+            }
+            if (nn != 4) tet = false;
+            L.cnperm[d] = perm;
+        }
+        if (!tet) {
+            L.cnode.clear();
+            L.cnperm.clear();
+        }
+    }
+
     // ---- state, device order (ghosts unchanged)
     L.q.resize(5 * (size_t)n);
     const double* src[5] = {v.rho, v.u, v.v, v.w, v.p};

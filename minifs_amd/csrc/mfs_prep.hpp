@@ -39,6 +39,11 @@ struct HostLayout {
     std::vector<double> fgeo;                  // 4*nF: xfn, yfn, zfn, area
     std::vector<double> dn1, dn2;              // nF: |(x_fc - x_cc) . n| for each side
     std::vector<int> fno, fn;                  // face->node CSR in device node ids
+    // tetrahedral meshes: the 4 nodes of each device cell (device node ids)
+    // and, per face slot s, 6 bits at 6s: the positions in that list of the
+    // face's 3 nodes in the face's own node order (empty otherwise)
+    std::vector<int> cnode;
+    std::vector<int> cnperm;
     std::vector<int> btype;                    // nB, by boundary face (ghost) index k
     // cells: device face ids in ascending mesh face id (CalC2F,
     // grid_polyhedra_auxiliary_functions.cpp:1345-1399)

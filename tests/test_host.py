@@ -221,6 +221,19 @@ def test_preprocess_matches_oracle_on_config1(tmp_path):
         np.testing.assert_array_equal(g.array(nm), o.get(nm), err_msg=nm)
 
 
+def test_wall_force_of_a_uniform_pressure():
+    """Divergence theorem over the bump wall, a graph surface over the unit
+    square: a uniform pressure gives F = (0, 0, -(p + p_bar)) exactly up to
+    rounding, whatever the bump."""
+    p = mfs.make_params(0.8, 1.25, 10.0)
+    for bump in (0.0, 0.05, 0.08):
+        g = mfs.Grid.box_bump(9, bump_height=bump).decouple().preprocess(p)
+        pf = np.full(g.info["nTCell"] + g.info["nBFace"], 123.0)
+        F, lift = mfs.wall_force(g, pf, p)
+        np.testing.assert_allclose(F, [0.0, 0.0, -(123.0 + p.p_bar)], rtol=0, atol=1e-8 * p.p_bar)
+        assert abs(lift - F[2] * np.cos(np.deg2rad(1.25))) < 1e-9 * p.p_bar
+
+
 def test_solver_without_gpu_fails_loudly():
     """No silent CPU fallback: creating a device context needs a GPU."""
     from tests.conftest import gpu_available
