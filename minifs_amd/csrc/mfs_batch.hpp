@@ -29,8 +29,9 @@ struct BatchSolve {
     FlowConst K;
     // gradient
     const int* nlo;
-    const int2* nlab;
-    const R* nlw;
+    const int* ell_off;
+    const int2* ell_ab;
+    const R* ell_w;
     const double* nden;
     R* qn;
     const int *fno, *fn, *btype;
@@ -38,6 +39,9 @@ struct BatchSolve {
     R *gx, *gy, *gz;
     const int4* cnode;
     const int* cnperm;
+    // shared-face residual blocks (tetrahedral meshes)
+    const int *fb_cell, *fb_fo, *fb_face;
+    const unsigned short* fb_slot;
     // sweeps
     SweepArgs<R> sw;
 };
@@ -48,8 +52,16 @@ __global__ __launch_bounds__(kBlock) void k_b_flux_ts(const BatchSolve<R>* __res
     const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;  // XCD-aware inside the solve
-    flux_ts_body<MAXF, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn, S.q, S.vol,
-                              S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min, S.q32, S.diag32);
+    if constexpr (MAXF == 4) {
+        extern __shared__ double sh_faces[];
+        flux_ts_shared_body<MIXED>(lg, S.fb_cell, S.fb_fo, S.fb_face, S.fb_slot, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn,
+                                   S.q, S.vol, S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min,
+                                   S.q32, S.diag32, sh_faces);
+    } else {
+        flux_ts_body<MAXF, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn, S.q,
+                                  S.vol, S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min, S.q32,
+                                  S.diag32);
+    }
 }
 
 // residual norms, dt extrema and the history record of every solve (one
@@ -93,7 +105,8 @@ __global__ __launch_bounds__(kBlock) void k_b_node(const BatchSolve<R>* __restri
     const BatchSolve<R>& S = T[node_solve[blockIdx.x]];
     const int lb = (int)blockIdx.x - S.blk0_node;
     const int lg = (lb % kXcd) * (S.nblk_node / kXcd) + lb / kXcd;
-    node_body<R>(lg * kBlock + (int)threadIdx.x, S.nN, S.nlo, S.nlab, S.nlw, S.nden, S.qnew, S.qn);
+    node_body<R>(lg * kBlock + (int)threadIdx.x, lg * (kBlock / 64) + (int)threadIdx.x / 64, S.nN, S.nlo, S.ell_off,
+                 S.ell_ab, S.ell_w, S.nden, S.qnew, S.qn);
 }
 
 template <int MAXF, typename R>

@@ -54,7 +54,7 @@ struct mfs_ctx {
     float* dq32 = nullptr;    // final DQ incl. ghosts (0)
     float *gx32 = nullptr, *gy32 = nullptr, *gz32 = nullptr;
     float* qn32 = nullptr;
-    float* nlw32 = nullptr;   // node weights
+    float* ell_w32 = nullptr; // node weights
     float4* fgeo32 = nullptr; // unit normal + area
     // faces (device order)
     int2* fcell = nullptr;
@@ -63,13 +63,18 @@ struct mfs_ctx {
     int *fno = nullptr, *fn = nullptr, *btype = nullptr;
     // cells
     int *cfo = nullptr, *cf = nullptr;
+    int *fb_cell = nullptr, *fb_fo = nullptr, *fb_face = nullptr;  // shared-face residual blocks (tets)
+    unsigned short* fb_slot = nullptr;
+    int fb_maxu = 0;
     int4* cnode = nullptr;   // tetrahedral meshes: 4 nodes per cell
     int* cnperm = nullptr;   // and each face's node positions
     double* vol = nullptr;
     // node gather
     int* nlo = nullptr;
-    int2* nlab = nullptr;
-    double *nlw = nullptr, *nden = nullptr;
+    int* ell_off = nullptr;   // per wave of 64 nodes: first entry (entry-major rows of 64)
+    int2* ell_ab = nullptr;
+    double* ell_w = nullptr;
+    double* nden = nullptr;
     int nsym = 0;
     int* sym_node = nullptr;
     double* sym_n = nullptr;

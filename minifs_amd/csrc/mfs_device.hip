@@ -182,6 +182,20 @@ bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
 template <int MAXF>
 bool launch_flux_ts_t(mfs_ctx* ctx, double cfl, double cfl_min) {
     Timed t(ctx, "flux_ts");
+    if (MAXF == 4 && ctx->fb_cell) {
+        const size_t lds = sizeof(double) * 7 * (size_t)std::max(ctx->fb_maxu, 1);
+        if (ctx->precision == MFS_MIXED)
+            hipLaunchKernelGGL(dev::k_flux_ts_shared<true>, dim3(ctx->nblk_cell), dim3(kBlock), lds, ctx->stream,
+                               ctx->fb_cell, ctx->fb_fo, ctx->fb_face, ctx->fb_slot, ctx->cfo, ctx->cf, ctx->fcell,
+                               ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res, ctx->dt, ctx->diag, ctx->part,
+                               ctx->K, cfl, cfl_min, ctx->q32, ctx->diag32);
+        else
+            hipLaunchKernelGGL(dev::k_flux_ts_shared<false>, dim3(ctx->nblk_cell), dim3(kBlock), lds, ctx->stream,
+                               ctx->fb_cell, ctx->fb_fo, ctx->fb_face, ctx->fb_slot, ctx->cfo, ctx->cf, ctx->fcell,
+                               ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res, ctx->dt, ctx->diag, ctx->part,
+                               ctx->K, cfl, cfl_min, nullptr, nullptr);
+        return check_launch(ctx, "k_flux_ts_shared");
+    }
     if (ctx->precision == MFS_MIXED)
         hipLaunchKernelGGL((dev::k_flux_ts<MAXF, true>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
                            ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res,
@@ -289,19 +303,19 @@ bool launch_gradient_t(mfs_ctx* ctx) {
     const double* q = ctx->q[ctx->cur];
     constexpr bool F64 = sizeof(R) == sizeof(double);
     R* qn;
-    const R* nlw;
+    const R* ew;
     const typename dev::Real<R>::vec4* fgeo;
     R *gx, *gy, *gz;
     if constexpr (F64) {
         qn = ctx->qn;
-        nlw = ctx->nlw;
+        ew = ctx->ell_w;
         fgeo = ctx->fgeo;
         gx = ctx->gx;
         gy = ctx->gy;
         gz = ctx->gz;
     } else {
         qn = ctx->qn32;
-        nlw = ctx->nlw32;
+        ew = ctx->ell_w32;
         fgeo = ctx->fgeo32;
         gx = ctx->gx32;
         gy = ctx->gy32;
@@ -310,7 +324,7 @@ bool launch_gradient_t(mfs_ctx* ctx) {
     {
         Timed t(ctx, "node");
         hipLaunchKernelGGL(dev::k_node<R>, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo,
-                           ctx->nlab, nlw, ctx->nden, q, qn);
+                           ctx->ell_off, ctx->ell_ab, ew, ctx->nden, q, qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
             hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, ctx->stream, ctx->nsym,
@@ -457,10 +471,37 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     }
     if (!L.cnode.empty())
         ok = ok && dalloc(ctx, ctx->cnode, (size_t)L.nC, "cnode") && dalloc(ctx, ctx->cnperm, (size_t)L.nC, "cnperm");
+    if (!L.fb_cell.empty()) {
+        if ((size_t)ctx->nblk_cell * kBlock != L.fb_cell.size()) {
+            set_error("mfs_ctx_create: residual workgroup table does not match the cell grid");
+            ok = false;
+        }
+        ctx->fb_maxu = L.fb_maxu;
+        ok = ok && dalloc(ctx, ctx->fb_cell, L.fb_cell.size(), "fb_cell") &&
+             dalloc(ctx, ctx->fb_fo, L.fb_fo.size(), "fb_fo") && dalloc(ctx, ctx->fb_face, L.fb_face.size(), "fb_face") &&
+             dalloc(ctx, ctx->fb_slot, L.fb_slot.size(), "fb_slot");
+    }
     ok = ok && dalloc(ctx, ctx->cfo, L.nC + 1, "cfo") && dalloc(ctx, ctx->cf, L.cf.size(), "cf") &&
          dalloc(ctx, ctx->vol, L.nC, "vol");
-    ok = ok && dalloc(ctx, ctx->nlo, L.nN + 1, "nlo") && dalloc(ctx, ctx->nlab, L.nla.size(), "nlab") &&
-         dalloc(ctx, ctx->nlw, L.nlw.size(), "nlw") && dalloc(ctx, ctx->nden, L.nN, "nden");
+    // node entries, entry-major per wave of 64 nodes (padding: cell 0, skipped)
+    const int nwave = (int)nblk_xcd(L.nN) * (kBlock / 64);
+    std::vector<int> ell_off((size_t)nwave + 1, 0);
+    for (int w = 0; w < nwave; ++w) {
+        int K = 0;
+        for (int p = 64 * w; p < std::min(L.nN, 64 * w + 64); ++p) K = std::max(K, L.nlo[p + 1] - L.nlo[p]);
+        ell_off[w + 1] = ell_off[w] + 64 * K;
+    }
+    std::vector<int2> ell_ab((size_t)ell_off[nwave], make_int2(0, -1));
+    std::vector<double> ell_w((size_t)ell_off[nwave], 0.0);
+    for (int p = 0; p < L.nN; ++p)
+        for (int k = 0; k < L.nlo[p + 1] - L.nlo[p]; ++k) {
+            const size_t e = (size_t)ell_off[p / 64] + 64 * (size_t)k + (p % 64);
+            ell_ab[e] = make_int2(L.nla[L.nlo[p] + k], L.nlb[L.nlo[p] + k]);
+            ell_w[e] = L.nlw[L.nlo[p] + k];
+        }
+    ok = ok && dalloc(ctx, ctx->nlo, L.nN + 1, "nlo") && dalloc(ctx, ctx->ell_off, ell_off.size(), "ell_off") &&
+         dalloc(ctx, ctx->ell_ab, ell_ab.size(), "ell_ab") && dalloc(ctx, ctx->ell_w, ell_w.size(), "ell_w") &&
+         dalloc(ctx, ctx->nden, L.nN, "nden");
     ctx->nsym = (int)L.sym_node.size();
     ok = ok && dalloc(ctx, ctx->sym_node, L.sym_node.size(), "sym_node") &&
          dalloc(ctx, ctx->sym_n, L.sym_n.size(), "sym_n");
@@ -474,7 +515,7 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              dalloc(ctx, ctx->dqf32, 5 * (size_t)L.nC, "dq_forward32") && dalloc(ctx, ctx->dq32, 5 * n, "dq32") &&
              dalloc(ctx, ctx->gx32, 5 * n, "dqdx32") && dalloc(ctx, ctx->gy32, 5 * n, "dqdy32") &&
              dalloc(ctx, ctx->gz32, 5 * n, "dqdz32") && dalloc(ctx, ctx->qn32, 5 * (size_t)L.nN, "qnode32") &&
-             dalloc(ctx, ctx->nlw32, L.nlw.size(), "nlw32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32");
+             dalloc(ctx, ctx->ell_w32, ell_w.size(), "ell_w32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32");
     }
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err");
@@ -497,8 +538,6 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
             fc[i] = make_int2(L.c1[i], L.c2[i]);
             fd[i] = make_double2(L.dn1[i], L.dn2[i]);
         }
-        std::vector<int2> nab(L.nla.size());
-        for (size_t k = 0; k < L.nla.size(); ++k) nab[k] = make_int2(L.nla[k], L.nlb[k]);
         std::vector<double> qa(5 * n);
         for (size_t c = 0; c < n; ++c)
             for (int j = 0; j < 5; ++j) qa[c * 5 + j] = L.q[(size_t)j * n + c];
@@ -510,13 +549,19 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              upload(ctx, ctx->btype, L.btype.data(), L.nB, "upload btype") &&
              upload(ctx, ctx->cfo, L.cfo.data(), L.nC + 1, "upload cfo") && upload(ctx, ctx->cf, L.cf.data(), L.cf.size(), "upload cf") &&
              upload(ctx, ctx->vol, L.vol.data(), L.nC, "upload vol") && upload(ctx, ctx->nlo, L.nlo.data(), L.nN + 1, "upload nlo") &&
-             upload(ctx, ctx->nlab, nab.data(), nab.size(), "upload nlab") &&
-             upload(ctx, ctx->nlw, L.nlw.data(), L.nlw.size(), "upload nlw") &&
+             upload(ctx, ctx->ell_off, ell_off.data(), ell_off.size(), "upload ell_off") &&
+             upload(ctx, ctx->ell_ab, ell_ab.data(), ell_ab.size(), "upload ell_ab") &&
+             upload(ctx, ctx->ell_w, ell_w.data(), ell_w.size(), "upload ell_w") &&
              upload(ctx, ctx->nden, L.nden.data(), L.nN, "upload nden") &&
              upload(ctx, ctx->sym_node, L.sym_node.data(), L.sym_node.size(), "upload sym") &&
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
              upload(ctx, ctx->chunk, reinterpret_cast<const int2*>(L.chunk.data()), (size_t)ctx->nchunk, "upload chunk") &&
              upload(ctx, ctx->cfs, L.cfs.data(), L.cfs.size(), "upload cfs") &&
+             (L.fb_cell.empty() ||
+              (upload(ctx, ctx->fb_cell, L.fb_cell.data(), L.fb_cell.size(), "upload fb_cell") &&
+               upload(ctx, ctx->fb_fo, L.fb_fo.data(), L.fb_fo.size(), "upload fb_fo") &&
+               upload(ctx, ctx->fb_face, L.fb_face.data(), L.fb_face.size(), "upload fb_face") &&
+               upload(ctx, ctx->fb_slot, L.fb_slot.data(), L.fb_slot.size(), "upload fb_slot"))) &&
              (L.cnode.empty() ||
               (upload(ctx, ctx->cnode, reinterpret_cast<const int4*>(L.cnode.data()), (size_t)L.nC, "upload cnode") &&
                upload(ctx, ctx->cnperm, L.cnperm.data(), (size_t)L.nC, "upload cnperm"))) &&
@@ -535,10 +580,10 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
             for (int i = 0; i < L.nF; ++i)
                 g32[i] = make_float4((float)L.fgeo[4 * (size_t)i], (float)L.fgeo[4 * (size_t)i + 1],
                                      (float)L.fgeo[4 * (size_t)i + 2], (float)L.fgeo[4 * (size_t)i + 3]);
-            f32.resize(std::max(L.nlw.size(), 5 * n));
-            for (size_t k = 0; k < L.nlw.size(); ++k) f32[k] = (float)L.nlw[k];
+            f32.resize(std::max(ell_w.size(), 5 * n));
+            for (size_t k = 0; k < ell_w.size(); ++k) f32[k] = (float)ell_w[k];
             ok = upload(ctx, ctx->fgeo32, g32.data(), L.nF, "upload fgeo32") &&
-                 upload(ctx, ctx->nlw32, f32.data(), L.nlw.size(), "upload nlw32");
+                 upload(ctx, ctx->ell_w32, f32.data(), ell_w.size(), "upload ell_w32");
             for (size_t k = 0; k < 5 * n; ++k) f32[k] = (float)qa[k];
             ok = ok && upload(ctx, ctx->q32, f32.data(), 5 * n, "upload q32");
             // hipMemcpyAsync reads the host buffers later: wait before they go out of scope
@@ -1012,13 +1057,18 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     E.diag32 = c->diag32;
     E.K = c->K;
     E.nlo = c->nlo;
-    E.nlab = c->nlab;
+    E.ell_off = c->ell_off;
+    E.ell_ab = c->ell_ab;
     E.nden = c->nden;
     E.fno = c->fno;
     E.fn = c->fn;
     E.btype = c->btype;
     E.cnode = c->cnode;
     E.cnperm = c->cnperm;
+    E.fb_cell = c->fb_cell;
+    E.fb_fo = c->fb_fo;
+    E.fb_face = c->fb_face;
+    E.fb_slot = c->fb_slot;
     auto& A = E.sw;
     A.nC = c->nC;
     A.nchunks = c->nchunk;
@@ -1035,7 +1085,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     A.cell0 = c->cell0;
     A.err = c->err;
     if constexpr (sizeof(R) == sizeof(double)) {
-        E.nlw = c->nlw;
+        E.ell_w = c->ell_w;
         E.qn = c->qn;
         E.fgeo_r = c->fgeo;
         E.gx = c->gx;
@@ -1047,7 +1097,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
         A.dqf = c->dqf;
         A.dqb = c->dq;
     } else {
-        E.nlw = c->nlw32;
+        E.ell_w = c->ell_w32;
         E.qn = c->qn32;
         E.fgeo_r = c->fgeo32;
         E.gx = c->gx32;
@@ -1116,12 +1166,16 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         return MFS_ERR_HIP;
     b->table_pending = true;
     auto* D = static_cast<dev::BatchSolve<R>*>(b->table_dev);
+    size_t lds_flux = 0;  // shared-face residual: 7 doubles per distinct face of the largest block
+    if (MAXF == 4)
+        for (int a = 0; a < ns; ++a)
+            lds_flux = std::max(lds_flux, sizeof(double) * 7 * (size_t)std::max(b->slots[b->active[a]]->fb_maxu, 1));
     const dim3 sweep_grid((unsigned)std::min(b->sweep_grid, b->nitems));
     for (int i = 0; i < nsteps; ++i) {
         const dev::BatchSolve<R>* Tf = D + (size_t)i * 2 * ns;
         const dev::BatchSolve<R>* Tb = Tf + ns;
-        hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
-                           b->cell_solve);
+        hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), lds_flux, b->stream,
+                           Tf, b->cell_solve);
         hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(kBlock), 0, b->stream, Tf);
         hipLaunchKernelGGL((dev::k_b_clamp<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                            b->cell_solve);

@@ -346,6 +346,136 @@ __global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restric
                               diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
 }
 
+// The same residual, time step and Diag for tetrahedral meshes, with the
+// Roe flux of each face evaluated once per workgroup: workgroup b takes the
+// 256 cells with mesh ids 256b..256b+255 (a compact region in TDTree
+// numbering, where about two thirds of the cells' face slots are distinct
+// faces), evaluates the flux of each distinct face into LDS with the
+// pressures of both sides, then every cell adds its faces' fluxes in its
+// face order.  Inputs of each flux (q[c1], q[c2], the face normal and area)
+// and every sum are those of flux_ts_body, so the results are the same bits.
+// sh: 7 doubles per distinct face (flux, p of c1, p of c2).
+template <bool MIXED>
+__device__ __forceinline__ void flux_ts_shared_body(int b, const int* __restrict__ fb_cell, const int* __restrict__ fb_fo,
+                                                    const int* __restrict__ fb_face,
+                                                    const unsigned short* __restrict__ fb_slot,
+                                                    const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                    const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                                    const double2* __restrict__ fdn, const double* __restrict__ q,
+                                                    const double* __restrict__ vol, double* __restrict__ res,
+                                                    double* __restrict__ dt, double* __restrict__ diag,
+                                                    double* __restrict__ pblk, const FlowConst& K, double cfl,
+                                                    double cfl_min, float* __restrict__ q32,
+                                                    float* __restrict__ diag32, double* sh) {
+    const int f0 = fb_fo[b], nu = fb_fo[b + 1] - f0;
+    for (int k = (int)threadIdx.x; k < nu; k += kBlock) {
+        const int f = fb_face[f0 + k];
+        const int2 cc = fcell[f];
+        const double4 g = fgeo[f];
+        double ql[5], qr[5], F[5];
+        load5(q + (size_t)cc.x * 5, ql);
+        load5(q + (size_t)cc.y * 5, qr);
+        roe_flux<double>(ql, qr, g.x, g.y, g.z, g.w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) sh[k * 7 + j] = F[j];
+        sh[k * 7 + 5] = ql[4];
+        sh[k * 7 + 6] = qr[4];
+    }
+    __syncthreads();
+    const int c = fb_cell[(size_t)b * kBlock + threadIdx.x];
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double vmin = kBIGd, vmax = 0.0;
+    if (c >= 0) {
+        double qc[5];
+        load5(q + (size_t)c * 5, qc);
+        int fs[4];
+        cell_faces<4>(cfo, cf, c, fs);
+        int2 cc[4];
+        double4 g[4];
+        double2 d[4];
+        int sl[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            cc[s] = fcell[fs[s]];
+            g[s] = fgeo[fs[s]];
+            d[s] = fdn[fs[s]];
+            sl[s] = fb_slot[(size_t)c * 4 + s];
+        }
+        const double c2tmp = K.gam * (qc[4] + K.p_bar) / qc[0];
+        const double a = sqrt(c2tmp);
+        double dtc = kBIGd;
+        int det = 1;
+        double eig[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bool first = (cc[s].x == c);
+            const double* e = sh + sl[s] * 7;
+            const double dn = first ? d[s].x : d[s].y;
+            double vn = qc[1] * g[s].x + qc[2] * g[s].y + qc[3] * g[s].z;
+            vn = fabs(vn);
+            eig[s] = vn + a;
+            const double pl = e[5], pr = e[6];
+            const double dp = fabs(pr - pl) / (pr + pl + K.p_bar + K.p_bar);
+            if (first) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) r[j] -= e[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) r[j] += e[j];
+            }
+            dtc = smin(dtc, dn / eig[s]);
+            if (dp > 0.1) det = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) res[(size_t)c * 5 + j] = r[j];
+        const double p = qc[4];
+        double cfl_tmp;
+        if (p > K.p_break)
+            cfl_tmp = cfl;
+        else if (p < K.p_min)
+            cfl_tmp = cfl_min;
+        else
+            cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
+        if (!det) cfl_tmp = cfl_min;
+        dtc *= cfl_tmp;
+        dt[c] = dtc;
+        vmin = dtc;
+        vmax = dtc;
+        double dg = vol[c] / dtc;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
+        diag[c] = dg;
+        if (MIXED) {
+            diag32[c] = (float)dg;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) q32[(size_t)c * 5 + j] = (float)qc[j];
+        }
+    }
+    double s2[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
+    block_sum<5>(s2, pblk);
+    block_minmax(vmin, vmax, pblk + 5, pblk + 6);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_flux_ts_shared(const int* __restrict__ fb_cell, const int* __restrict__ fb_fo,
+                                                           const int* __restrict__ fb_face,
+                                                           const unsigned short* __restrict__ fb_slot,
+                                                           const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                           const int2* __restrict__ fcell,
+                                                           const double4* __restrict__ fgeo,
+                                                           const double2* __restrict__ fdn, const double* __restrict__ q,
+                                                           const double* __restrict__ vol, double* __restrict__ res,
+                                                           double* __restrict__ dt, double* __restrict__ diag,
+                                                           double* __restrict__ part, FlowConst K, double cfl,
+                                                           double cfl_min, float* __restrict__ q32,
+                                                           float* __restrict__ diag32) {
+    extern __shared__ double sh_faces[];
+    flux_ts_shared_body<MIXED>(xcd_block(), fb_cell, fb_fo, fb_face, fb_slot, cfo, cf, fcell, fgeo, fdn, q, vol, res, dt,
+                               diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32, sh_faces);
+}
+
 // Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
 // With `run`, only cells whose dt exceeds ratio_dtmax*dt_min are clamped
 // and recomputed (compute_timestep.cpp:929-945), and only when the
@@ -454,24 +584,31 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __res
 // Node values (CompNodeVar3D_dist, cal_gradient.cpp:1922-2372): each node
 // sums its weighted contributions in the reference's order, then divides by
 // WeightNode + TINY.
+// Entries are stored per wave of 64 consecutive nodes, entry-major
+// (entry k of the wave's lane i at ell_off[w] + 64k + i, padded to the
+// wave's largest count), so every entry load of a wave is one coalesced
+// 512-byte row; each node still adds its own entries in its order.
 template <typename R>
-__device__ __forceinline__ void node_body(int p, int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
-                                          const R* __restrict__ nlw, const double* __restrict__ nden,
+__device__ __forceinline__ void node_body(int p, int w, int nN, const int* __restrict__ nlo,
+                                          const int* __restrict__ ell_off, const int2* __restrict__ ell_ab,
+                                          const R* __restrict__ ell_w, const double* __restrict__ nden,
                                           const double* __restrict__ q, R* __restrict__ qn) {
-    if (p >= nN) return;
+    const int lane = (int)threadIdx.x & 63;
+    const int e0 = ell_off[w];
+    const int K = (ell_off[w + 1] - e0) / 64;  // wave-uniform trip count
+    const int cnt = p < nN ? nlo[p + 1] - nlo[p] : 0;
     R acc[5] = {R(0), R(0), R(0), R(0), R(0)};
-    const int e0 = nlo[p], e1 = nlo[p + 1];
     // entries in groups of U: the gathers of a group are issued together,
     // then added in entry order
     constexpr int U = 6;
-    for (int k = e0; k < e1; k += U) {
+    for (int k = 0; k < K; k += U) {
         int2 ab[U];
-        R w[U], qa[U][5];
+        R wt[U], qa[U][5];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
-            const int e = (k + i < e1) ? k + i : k;
-            ab[i] = nlab[e];
-            w[i] = nlw[e];
+            const int kk = (k + i < K) ? k + i : k;
+            ab[i] = ell_ab[(size_t)e0 + (size_t)kk * 64 + lane];
+            wt[i] = ell_w[(size_t)e0 + (size_t)kk * 64 + lane];
         }
 #pragma unroll
         for (int i = 0; i < U; ++i)
@@ -479,30 +616,34 @@ __device__ __forceinline__ void node_body(int p, int nN, const int* __restrict__
             for (int j = 0; j < 5; ++j) qa[i][j] = (R)q[(size_t)ab[i].x * 5 + j];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
-            if (k + i >= e1) break;
+            if (k + i >= cnt) break;
             if (ab[i].y < 0) {
 #pragma unroll
-                for (int j = 0; j < 5; ++j) acc[j] += qa[i][j] * w[i];
+                for (int j = 0; j < 5; ++j) acc[j] += qa[i][j] * wt[i];
             } else {
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
                     R fq = qa[i][j] + (R)q[(size_t)ab[i].y * 5 + j];
                     fq *= R(0.5);
-                    acc[j] += fq * w[i];
+                    acc[j] += fq * wt[i];
                 }
             }
         }
     }
+    if (p >= nN) return;
     const R d = (R)nden[p];
 #pragma unroll
     for (int j = 0; j < 5; ++j) qn[(size_t)p * 5 + j] = acc[j] / d;
 }
 
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__ nlo, const int2* __restrict__ nlab,
-                                                 const R* __restrict__ nlw, const double* __restrict__ nden,
-                                                 const double* __restrict__ q, R* __restrict__ qn) {
-    node_body<R>(xcd_block() * kBlock + (int)threadIdx.x, nN, nlo, nlab, nlw, nden, q, qn);
+__global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__ nlo, const int* __restrict__ ell_off,
+                                                 const int2* __restrict__ ell_ab, const R* __restrict__ ell_w,
+                                                 const double* __restrict__ nden, const double* __restrict__ q,
+                                                 R* __restrict__ qn) {
+    const int lb = xcd_block();
+    node_body<R>(lb * kBlock + (int)threadIdx.x, lb * (kBlock / 64) + (int)threadIdx.x / 64, nN, nlo, ell_off, ell_ab,
+                 ell_w, nden, q, qn);
 }
 
 // Symmetry-plane velocity projection of the node values (cal_gradient.cpp:2332-2360).

@@ -143,6 +143,43 @@ MFS_HD void flux_lusgs(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p
     for (int i = 0; i < 5; i++) flux[i] -= eig * DQ[i];
 }
 
+// FluxLUSGS3D in increment form, for the FP32 operator of mixed precision.
+// The reference's expression forms F(Q+DQ) and F(Q) separately and
+// subtracts them; in FP32 the energy terms ((Q4+p+peff)*v_n, peff =
+// gam*p_bar/gam1 ~ 3e5) leave a rounding error of ~1e-7*|F| that does not
+// shrink with DQ.  Expanding the difference algebraically gives the same
+// function with every term proportional to DQ:
+//   dvn = (DQ1..3.n - v_n*DQ0)/Q0',  Q' = Q + DQ
+//   drv2 = 0.5*(2*Q1..3.DQ1..3 + |DQ1..3|^2 - Q0*|u|^2*DQ0)/Q0'
+//   dp = gam1*(DQ4 - drv2)
+//   dF0 = DQ1..3.n,  dF_i = Q_i*dvn + DQ_i*v_n' + dp*n_i,  dF4 = (Q4+p+peff)*dvn + (DQ4+dp)*v_n'
+//   flux = dF - eig*DQ
+template <typename R>
+MFS_HD void flux_lusgs_delta(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga, R flux[5]) {
+    const R gam1 = gam - R(1.);
+    const R rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+    const R Q1 = rho * u, Q2 = rho * v, Q3 = rho * w;
+    const R vv = u * u + v * v + w * w;
+    const R Q4 = p / gam1 + R(0.5) * rho * vv;
+    const R v_n = u * nx + v * ny + w * nz;
+    const R eig = (fabs(v_n) + sqrt(gam * (p + p_bar) / rho)) * lhs_omga;
+    const R peff = gam * p_bar / gam1;
+    const R Q0n = rho + DQ[0];
+    const R dmn = DQ[1] * nx + DQ[2] * ny + DQ[3] * nz;  // increment of the mass flux
+    const R dvn = (dmn - v_n * DQ[0]) / Q0n;
+    const R vn1 = v_n + dvn;
+    const R qd = Q1 * DQ[1] + Q2 * DQ[2] + Q3 * DQ[3];
+    const R dd = DQ[1] * DQ[1] + DQ[2] * DQ[2] + DQ[3] * DQ[3];
+    const R drv2 = R(0.5) * (R(2.) * qd + dd - rho * vv * DQ[0]) / Q0n;
+    const R dp = gam1 * (DQ[4] - drv2);
+    flux[0] = dmn;
+    flux[1] = Q1 * dvn + DQ[1] * vn1 + dp * nx;
+    flux[2] = Q2 * dvn + DQ[2] * vn1 + dp * ny;
+    flux[3] = Q3 * dvn + DQ[3] * vn1 + dp * nz;
+    flux[4] = (Q4 + p + peff) * dvn + (DQ[4] + dp) * vn1;
+    for (int i = 0; i < 5; i++) flux[i] -= eig * DQ[i];
+}
+
 // The three-stage DQ positivity limiter (DQ_limit == 2) of both sweeps
 // (forward_lusgs.cpp:939-973 and 1184-1218).  q is the swept cell's state.
 template <typename R>

@@ -473,6 +473,43 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         }
     }
 
+    // ---- tetrahedral meshes: residual workgroups over mesh-order cell blocks
+    L.fb_cell.clear();
+    L.fb_fo.clear();
+    L.fb_face.clear();
+    L.fb_slot.clear();
+    L.fb_maxu = 0;
+    if (!L.cnode.empty()) {
+        const int B = 256;
+        const int nfb = (nC + B - 1) / B;
+        const int nblk = (nfb + 7) / 8 * 8;
+        L.fb_cell.assign((size_t)nblk * B, -1);
+        L.fb_fo.assign((size_t)nblk + 1, 0);
+        L.fb_slot.assign(4 * (size_t)nC, 0);
+        std::vector<int> stamp(nF, -1), idx(nF, 0);
+        for (int b = 0; b < nblk; ++b) {
+            int nu = 0;
+            for (int t = 0; t < B && b < nfb; ++t) {
+                const int m = b * B + t;
+                if (m >= nC) break;
+                const int d = dcell[m];
+                L.fb_cell[(size_t)b * B + t] = d;
+                for (int k = L.cfo[d]; k < L.cfo[d + 1]; ++k) {
+                    const int f = L.cf[k];
+                    if (stamp[f] != b) {
+                        stamp[f] = b;
+                        idx[f] = nu++;
+                        L.fb_face.push_back(f);
+                    }
+                    L.fb_slot[4 * (size_t)d + (k - L.cfo[d])] = (unsigned short)idx[f];
+                }
+            }
+            if (nu > 65535) return "residual workgroup with too many faces";
+            L.fb_maxu = std::max(L.fb_maxu, nu);
+            L.fb_fo[b + 1] = L.fb_fo[b] + nu;
+        }
+    }
+
     // ---- state, device order (ghosts unchanged)
     L.q.resize(5 * (size_t)n);
     const double* src[5] = {v.rho, v.u, v.v, v.w, v.p};

@@ -44,6 +44,14 @@ struct HostLayout {
     // face's 3 nodes in the face's own node order (empty otherwise)
     std::vector<int> cnode;
     std::vector<int> cnperm;
+    // tetrahedral meshes, residual workgroups: block b owns the cells with
+    // mesh ids 256b..256b+255 (fb_cell: device ids, -1 = none), the distinct
+    // faces they touch (fb_face[fb_fo[b]..fb_fo[b+1]), device ids) and, per
+    // device cell and face slot, that face's index in its block's list.
+    // Blocks are padded to a multiple of 8 (empty blocks).
+    std::vector<int> fb_cell, fb_fo, fb_face;
+    std::vector<unsigned short> fb_slot;
+    int fb_maxu = 0;
     std::vector<int> btype;                    // nB, by boundary face (ghost) index k
     // cells: device face ids in ascending mesh face id (CalC2F,
     // grid_polyhedra_auxiliary_functions.cpp:1345-1399)

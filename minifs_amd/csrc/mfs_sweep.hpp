@@ -234,7 +234,10 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     R p[5] = {R(0), R(0), R(0), R(0), R(0)};
     if (mine) {
         R fl[5];
-        flux_lusgs<R>(qnb, dv, fnx, fny, fnz, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga, fl);
+        if constexpr (sizeof(R) == sizeof(double))  // the reference's expression, bit for bit
+            flux_lusgs<R>(qnb, dv, fnx, fny, fnz, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga, fl);
+        else  // FP32 operator: the same function in increment form (no cancellation floor)
+            flux_lusgs_delta<R>(qnb, dv, fnx, fny, fnz, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga, fl);
         const R tmp = FWD ? R(0.5) * far_ : far_;
 #pragma unroll
         for (int j = 0; j < 5; ++j) p[j] = tmp * fl[j];

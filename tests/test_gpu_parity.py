@@ -80,7 +80,10 @@ def test_polyhedral_cells_bitwise_vs_reference(tmp_path, ref_driver, cells):
 
 def test_entry_points_match_step(tmp_path, ref_driver):
     """The per-kernel entry points (reference call order) and the fused
-    mfs_step produce the same bits."""
+    mfs_step produce the same bits in every field.  The residual norms are
+    tree sums over different workgroup partitions (the fused residual kernel
+    groups cells by mesh id), so they agree to rounding; the dt extrema are
+    exact."""
     p, g, ref, out = _case(tmp_path, ref_driver, 8, 0.8, 1.25, 10.0, 5, dumps=(5,))
     a = mfs.Solver(g, p)
     b = mfs.Solver(g, p)
@@ -89,6 +92,8 @@ def test_entry_points_match_step(tmp_path, ref_driver):
     for nm in STATE + ("res", "dt", "dq", "dqdx", "dqdy", "dqdz"):
         np.testing.assert_array_equal(a.field(nm), b.field(nm), err_msg=nm)
     _assert_state_equal(a, out, 5, g.info["nTCell"])
-    np.testing.assert_array_equal(a.history(), b.history())
+    ha, hb = a.history(), b.history()
+    np.testing.assert_array_equal(ha[:, 5:], hb[:, 5:])
+    np.testing.assert_allclose(ha[:, :5], hb[:, :5], rtol=HIST_RTOL, atol=0)
     a.close()
     b.close()
