@@ -141,6 +141,16 @@ int mfs_grid_view(const mfs_grid* g, mfs_mesh_view* view);
 typedef struct mfs_ctx mfs_ctx;
 
 int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, int precision, mfs_ctx** out);
+
+/* Layout and launch options (no counterpart in the reference): results do
+ * not depend on them, only speed.  Zero fields take the defaults. */
+typedef struct mfs_ctx_options {
+    int tile_cells;     /* cells per storage tile of the LU-SGS sweeps (>= 64; default 4096) */
+    int sweep_waves;    /* waves per sweep workgroup: 4, 8 or 16 (default 4) */
+    int sweep_by_tiles; /* 0: flat level-order schedule (default); 1: one workgroup per tile */
+} mfs_ctx_options;
+int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p, int precision, const mfs_ctx_options* opt,
+                      mfs_ctx** out);
 int mfs_ctx_destroy(mfs_ctx* ctx);
 
 /* The kernel entry points, one per reference function. */
@@ -196,6 +206,7 @@ typedef struct mfs_ctx_info {
     int precision;
     int sweep_levels;      /* DAG depth of the LU-SGS sweep order */
     size_t device_bytes;   /* device memory held by the context */
+    int tile_cells, sweep_waves, sweep_tiles, sweep_workgroups, sweep_by_tiles;
 } mfs_ctx_info;
 int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
 /* Kernel time accounting: when enabled, every launch of the named kernel

@@ -56,7 +56,13 @@ class MfsScalars(C.Structure):
 
 class MfsCtxInfo(C.Structure):
     _fields_ = [("nTCell", C.c_int), ("nTFace", C.c_int), ("nBFace", C.c_int), ("nTNode", C.c_int),
-                ("precision", C.c_int), ("sweep_levels", C.c_int), ("device_bytes", C.c_size_t)]
+                ("precision", C.c_int), ("sweep_levels", C.c_int), ("device_bytes", C.c_size_t),
+                ("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_tiles", C.c_int),
+                ("sweep_workgroups", C.c_int), ("sweep_by_tiles", C.c_int)]
+
+
+class MfsCtxOptions(C.Structure):
+    _fields_ = [("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_by_tiles", C.c_int)]
 
 
 # Every symbol declared in include/mfs.h, with its ctypes signature.
@@ -78,6 +84,8 @@ _SIGS = {
     "mfs_grid_destroy": (None, [C.c_void_p]),
     "mfs_grid_view": (C.c_int, [C.c_void_p, C.POINTER(MfsMeshView)]),
     "mfs_ctx_create": (C.c_int, [C.POINTER(MfsMeshView), C.POINTER(MfsParams), C.c_int, C.POINTER(C.c_void_p)]),
+    "mfs_ctx_create_ex": (C.c_int, [C.POINTER(MfsMeshView), C.POINTER(MfsParams), C.c_int,
+                                    C.POINTER(MfsCtxOptions), C.POINTER(C.c_void_p)]),
     "mfs_ctx_destroy": (C.c_int, [C.c_void_p]),
     "mfs_zero_residuals": (C.c_int, [C.c_void_p]),
     "mfs_inviscid_flux": (C.c_int, [C.c_void_p]),

@@ -5,10 +5,12 @@
 // condition and device arrays) with one launch per kernel for all of them.
 // Each launch reads a table with one BatchSolve entry per solve; the cell
 // and node kernels map their workgroups to solves through a block map (each
-// solve owns a whole number of XCD groups of 8 workgroups), and the sweeps
-// walk a list of (solve, tile) items.  The per-cell arithmetic is the
+// solve owns a whole number of XCD groups of 8 workgroups).  The per-cell arithmetic is the
 // single-solve kernels' own (the same device functions), so a solve gives
-// the same bits alone and in a batch.
+// the same bits alone and in a batch.  The sweeps walk a list of (solve, u)
+// items, the u-th chunk in level order of every solve for u = 0, 1, ...;
+// dependencies never cross solves and each solve's chunks keep their order,
+// so the deadlock argument of the single-solve sweep holds.
 
 template <typename R>
 struct BatchSolve {
@@ -37,11 +39,6 @@ struct BatchSolve {
     const int *fno, *fn, *btype;
     const typename Real<R>::vec4* fgeo_r;
     R *gx, *gy, *gz;
-    const int4* cnode;
-    const int* cnperm;
-    // shared-face residual blocks (tetrahedral meshes)
-    const int *fb_cell, *fb_fo, *fb_face;
-    const unsigned short* fb_slot;
     // sweeps
     SweepArgs<R> sw;
 };
@@ -52,16 +49,8 @@ __global__ __launch_bounds__(kBlock) void k_b_flux_ts(const BatchSolve<R>* __res
     const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;  // XCD-aware inside the solve
-    if constexpr (MAXF == 4) {
-        extern __shared__ double sh_faces[];
-        flux_ts_shared_body<MIXED>(lg, S.fb_cell, S.fb_fo, S.fb_face, S.fb_slot, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn,
-                                   S.q, S.vol, S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min,
-                                   S.q32, S.diag32, sh_faces);
-    } else {
-        flux_ts_body<MAXF, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn, S.q,
-                                  S.vol, S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min, S.q32,
-                                  S.diag32);
-    }
+    flux_ts_body<MAXF, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn, S.q, S.vol,
+                              S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min, S.q32, S.diag32);
 }
 
 // residual norms, dt extrema and the history record of every solve (one
@@ -85,17 +74,16 @@ __global__ __launch_bounds__(kBlock) void k_b_clamp(const BatchSolve<R>* __restr
 template <bool FWD, int LPC, typename R>
 __global__ __launch_bounds__(kSweepBlock) void k_b_sweep(const BatchSolve<R>* __restrict__ T,
                                                         const int2* __restrict__ items, int nitems) {
+    // items: (solve, u) with u the position in that solve's level order;
+    // each wave walks every G-th item, in order
     const int W = (int)blockDim.x / 64;
-    const int w = (int)threadIdx.x / 64;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int2 sk = items[it];  // (solve, tile), tiles in sweep order within a solve
-        const SweepArgs<R>& A = T[sk.x].sw;
+    const int G = (int)gridDim.x * W;
+    for (int it = (int)blockIdx.x * W + (int)threadIdx.x / 64; it < nitems; it += G) {
+        const int2 su = items[it];
+        const SweepArgs<R>& A = T[su.x].sw;
         if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
-        const int c0 = A.tile_chunk[sk.y], c1 = A.tile_chunk[sk.y + 1];
-        for (int j = w; j < c1 - c0; j += W) {
-            const int t = FWD ? c0 + j : c1 - 1 - j;
-            if (!sweep_chunk<FWD, LPC, R>(A, T[sk.x].K, t)) break;
-        }
+        const int t = A.run[FWD ? su.y : A.nchunks - 1 - su.y];
+        if (!sweep_chunk<FWD, LPC, R>(A, T[su.x].K, t)) continue;
     }
 }
 
@@ -116,5 +104,5 @@ __global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restri
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
     grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
-                       S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz, S.cnode, S.cnperm);
+                       S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
 }

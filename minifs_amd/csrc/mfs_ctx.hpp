@@ -30,6 +30,7 @@ struct mfs_ctx {
     int nC = 0, nF = 0, nB = 0, nN = 0;
     int precision = MFS_FP64;
     int maxf = 4;  // face slots per cell used by the kernels (4 or 6)
+    int tile_cells = 4096, sweep_waves = 8;  // layout / launch options
     mfs_params P{};
     mfs::FlowConst K{};
     bool broken = false;
@@ -63,11 +64,6 @@ struct mfs_ctx {
     int *fno = nullptr, *fn = nullptr, *btype = nullptr;
     // cells
     int *cfo = nullptr, *cf = nullptr;
-    int *fb_cell = nullptr, *fb_fo = nullptr, *fb_face = nullptr;  // shared-face residual blocks (tets)
-    unsigned short* fb_slot = nullptr;
-    int fb_maxu = 0;
-    int4* cnode = nullptr;   // tetrahedral meshes: 4 nodes per cell
-    int* cnperm = nullptr;   // and each face's node positions
     double* vol = nullptr;
     // node gather
     int* nlo = nullptr;
@@ -83,6 +79,8 @@ struct mfs_ctx {
     // sweeps
     int2* chunk = nullptr;  // per chunk (device order): first device cell, cell count
     int* tile_chunk = nullptr;  // ntiles+1: first chunk of each tile
+    int* chunk_order = nullptr; // chunks in forward level order (flat schedule)
+    int sweep_by_tiles = 0;
     int ntiles = 0;
     int *rep_f = nullptr, *rep_b = nullptr;
     int* cfs = nullptr;     // lpc face slots per device cell

@@ -28,7 +28,7 @@ namespace dev {
 constexpr int kBlock = 256;
 constexpr int kXcd = 8;  // XCDs of an MI355X; cell-kernel grids are a multiple of this
 constexpr int kSweepWave = 64;
-constexpr int kSweepBlock = 512;  // waves of a sweep workgroup x 64: one tile per workgroup
+constexpr int kSweepBlock = 1024;  // largest sweep workgroup (16 waves); one tile per workgroup
 constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wave
 
 #include "mfs_kernels.hpp"
@@ -182,20 +182,6 @@ bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
 template <int MAXF>
 bool launch_flux_ts_t(mfs_ctx* ctx, double cfl, double cfl_min) {
     Timed t(ctx, "flux_ts");
-    if (MAXF == 4 && ctx->fb_cell) {
-        const size_t lds = sizeof(double) * 7 * (size_t)std::max(ctx->fb_maxu, 1);
-        if (ctx->precision == MFS_MIXED)
-            hipLaunchKernelGGL(dev::k_flux_ts_shared<true>, dim3(ctx->nblk_cell), dim3(kBlock), lds, ctx->stream,
-                               ctx->fb_cell, ctx->fb_fo, ctx->fb_face, ctx->fb_slot, ctx->cfo, ctx->cf, ctx->fcell,
-                               ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res, ctx->dt, ctx->diag, ctx->part,
-                               ctx->K, cfl, cfl_min, ctx->q32, ctx->diag32);
-        else
-            hipLaunchKernelGGL(dev::k_flux_ts_shared<false>, dim3(ctx->nblk_cell), dim3(kBlock), lds, ctx->stream,
-                               ctx->fb_cell, ctx->fb_fo, ctx->fb_face, ctx->fb_slot, ctx->cfo, ctx->cf, ctx->fcell,
-                               ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res, ctx->dt, ctx->diag, ctx->part,
-                               ctx->K, cfl, cfl_min, nullptr, nullptr);
-        return check_launch(ctx, "k_flux_ts_shared");
-    }
     if (ctx->precision == MFS_MIXED)
         hipLaunchKernelGGL((dev::k_flux_ts<MAXF, true>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
                            ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res,
@@ -244,6 +230,8 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     A.nchunks = ctx->nchunk;
     A.ntiles = ctx->ntiles;
     A.tile_chunk = ctx->tile_chunk;
+    A.run = ctx->chunk_order;
+    A.flat = ctx->sweep_by_tiles ? 0 : 1;
     A.chunk = ctx->chunk;
     A.cfs = ctx->cfs;
     A.fcell = ctx->fcell;
@@ -266,8 +254,9 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     A.err = ctx->err;
     A.depmask = ctx->depmask;
     A.cell0 = ctx->cell0;
-    const dim3 block(dev::kSweepBlock);
-    const dim3 grid((unsigned)std::min(ctx->sweep_grid, A.ntiles));
+    const dim3 block(64 * ctx->sweep_waves);
+    const int units = A.flat ? (A.nchunks + ctx->sweep_waves - 1) / ctx->sweep_waves : A.ntiles;
+    const dim3 grid((unsigned)std::max(1, std::min(ctx->sweep_grid, units)));
     {
         Timed t(ctx, "fwd_sweep");
         A.rep = ctx->rep_f;
@@ -336,7 +325,7 @@ bool launch_gradient_t(mfs_ctx* ctx) {
         Timed t(ctx, "gradient");
         hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
                            ctx->cfo, ctx->cf, ctx->fcell, fgeo, ctx->fno, ctx->fn, ctx->btype, q, qn, ctx->vol, gx, gy,
-                           gz, ctx->cnode, ctx->cnperm);
+                           gz);
         if (!check_launch(ctx, "k_grad")) return false;
     }
     return true;
@@ -381,12 +370,10 @@ void free_all(mfs_ctx* ctx) {
 }
 
 template <int LPC, typename R>
-int sweep_occupancy(int* per_cu) {
+int sweep_occupancy(int* per_cu, int block) {
     int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, LPC, R>, dev::kSweepBlock, 0) !=
-            hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, LPC, R>, dev::kSweepBlock, 0) !=
-            hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_sweep<true, LPC, R>, block, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, dev::k_sweep<false, LPC, R>, block, 0) != hipSuccess)
         return -1;
     *per_cu = std::min(a, b);
     return 0;
@@ -400,8 +387,20 @@ int sweep_occupancy(int* per_cu) {
 using namespace mfs;
 
 extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, int precision, mfs_ctx** out) {
+    return mfs_ctx_create_ex(mesh, p, precision, nullptr, out);
+}
+
+extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p, int precision,
+                                 const mfs_ctx_options* opt, mfs_ctx** out) {
     if (!mesh || !p || !out) {
         set_error("mfs_ctx_create: null argument");
+        return MFS_ERR_ARG;
+    }
+    const int tile_cells = (opt && opt->tile_cells) ? opt->tile_cells : kDefaultTileCells;
+    const int sweep_waves = (opt && opt->sweep_waves) ? opt->sweep_waves : 4;
+    const int sweep_by_tiles = (opt && opt->sweep_by_tiles) ? 1 : 0;
+    if (tile_cells < 64 || (sweep_waves != 4 && sweep_waves != 8 && sweep_waves != 16)) {
+        set_error("mfs_ctx_create: tile_cells must be >= 64 and sweep_waves 4, 8 or 16");
         return MFS_ERR_ARG;
     }
     *out = nullptr;
@@ -410,7 +409,7 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
         return MFS_ERR_ARG;
     }
     HostLayout L;
-    std::string e = build_layout(*mesh, L);
+    std::string e = build_layout(*mesh, L, tile_cells);
     if (!e.empty()) {
         set_error("mfs_ctx_create: " + e);
         return MFS_ERR_MESH;
@@ -431,6 +430,9 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ctx->nB = L.nB;
     ctx->nN = L.nN;
     ctx->maxf = L.max_faces <= 4 ? 4 : 6;
+    ctx->tile_cells = tile_cells;
+    ctx->sweep_waves = sweep_waves;
+    ctx->sweep_by_tiles = sweep_by_tiles;
     ctx->precision = precision;
     ctx->P = *p;
     ctx->K.gam = p->gam;
@@ -465,21 +467,15 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ok = ok && dalloc(ctx, ctx->fcell, L.nF, "fcell") && dalloc(ctx, ctx->fgeo, L.nF, "fgeo") &&
          dalloc(ctx, ctx->fdn, L.nF, "fdn") && dalloc(ctx, ctx->fno, L.nF + 1, "fno") &&
          dalloc(ctx, ctx->fn, L.fn.size(), "fn") && dalloc(ctx, ctx->btype, L.nB, "btype");
-    if (ctx->maxf == 4 && L.cnode.empty()) {
-        set_error("mfs_ctx_create: tetrahedral mesh without cell node lists");
-        ok = false;
-    }
-    if (!L.cnode.empty())
-        ok = ok && dalloc(ctx, ctx->cnode, (size_t)L.nC, "cnode") && dalloc(ctx, ctx->cnperm, (size_t)L.nC, "cnperm");
-    if (!L.fb_cell.empty()) {
-        if ((size_t)ctx->nblk_cell * kBlock != L.fb_cell.size()) {
-            set_error("mfs_ctx_create: residual workgroup table does not match the cell grid");
-            ok = false;
-        }
-        ctx->fb_maxu = L.fb_maxu;
-        ok = ok && dalloc(ctx, ctx->fb_cell, L.fb_cell.size(), "fb_cell") &&
-             dalloc(ctx, ctx->fb_fo, L.fb_fo.size(), "fb_fo") && dalloc(ctx, ctx->fb_face, L.fb_face.size(), "fb_face") &&
-             dalloc(ctx, ctx->fb_slot, L.fb_slot.size(), "fb_slot");
+    if (ctx->maxf == 4) {
+        // four faces per cell means tetrahedra: the gradient kernel reads
+        // face f's three nodes at fn[3f..3f+2]
+        for (int f = 0; f <= L.nF; ++f)
+            if (L.fno[f] != 3 * f) {
+                set_error("mfs_ctx_create: a mesh of 4-face cells has a face that is not a triangle");
+                ok = false;
+                break;
+            }
     }
     ok = ok && dalloc(ctx, ctx->cfo, L.nC + 1, "cfo") && dalloc(ctx, ctx->cf, L.cf.size(), "cf") &&
          dalloc(ctx, ctx->vol, L.nC, "vol");
@@ -506,7 +502,7 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
     ok = ok && dalloc(ctx, ctx->sym_node, L.sym_node.size(), "sym_node") &&
          dalloc(ctx, ctx->sym_n, L.sym_n.size(), "sym_n");
     ok = ok && dalloc(ctx, ctx->chunk, (size_t)ctx->nchunk, "chunk") && dalloc(ctx, ctx->cfs, L.cfs.size(), "cfs") &&
-         dalloc(ctx, ctx->tile_chunk, L.tile_chunk.size(), "tile_chunk") &&
+         dalloc(ctx, ctx->tile_chunk, L.tile_chunk.size(), "tile_chunk") && dalloc(ctx, ctx->chunk_order, L.run.size(), "chunk_order") &&
          dalloc(ctx, ctx->rep_f, L.rep_f.size(), "rep_f") &&
          dalloc(ctx, ctx->rep_b, L.rep_b.size(), "rep_b") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
     const bool mixed = precision == MFS_MIXED;
@@ -524,8 +520,10 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
         // co-resident waves for the cooperative sweep launch
         int per_cu = 0, cus = 0;
         const bool mixed = ctx->precision == MFS_MIXED;
-        const int occ = ctx->lpc == 4 ? (mixed ? sweep_occupancy<4, float>(&per_cu) : sweep_occupancy<4, double>(&per_cu))
-                                      : (mixed ? sweep_occupancy<8, float>(&per_cu) : sweep_occupancy<8, double>(&per_cu));
+        const int blk = 64 * ctx->sweep_waves;
+        const int occ = ctx->lpc == 4
+                            ? (mixed ? sweep_occupancy<4, float>(&per_cu, blk) : sweep_occupancy<4, double>(&per_cu, blk))
+                            : (mixed ? sweep_occupancy<8, float>(&per_cu, blk) : sweep_occupancy<8, double>(&per_cu, blk));
         ok = occ == 0 &&
              hip_ok(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
         if (!ok) set_error("mfs_ctx_create: occupancy query failed");
@@ -557,15 +555,8 @@ extern "C" int mfs_ctx_create(const mfs_mesh_view* mesh, const mfs_params* p, in
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
              upload(ctx, ctx->chunk, reinterpret_cast<const int2*>(L.chunk.data()), (size_t)ctx->nchunk, "upload chunk") &&
              upload(ctx, ctx->cfs, L.cfs.data(), L.cfs.size(), "upload cfs") &&
-             (L.fb_cell.empty() ||
-              (upload(ctx, ctx->fb_cell, L.fb_cell.data(), L.fb_cell.size(), "upload fb_cell") &&
-               upload(ctx, ctx->fb_fo, L.fb_fo.data(), L.fb_fo.size(), "upload fb_fo") &&
-               upload(ctx, ctx->fb_face, L.fb_face.data(), L.fb_face.size(), "upload fb_face") &&
-               upload(ctx, ctx->fb_slot, L.fb_slot.data(), L.fb_slot.size(), "upload fb_slot"))) &&
-             (L.cnode.empty() ||
-              (upload(ctx, ctx->cnode, reinterpret_cast<const int4*>(L.cnode.data()), (size_t)L.nC, "upload cnode") &&
-               upload(ctx, ctx->cnperm, L.cnperm.data(), (size_t)L.nC, "upload cnperm"))) &&
              upload(ctx, ctx->tile_chunk, L.tile_chunk.data(), L.tile_chunk.size(), "upload tile_chunk") &&
+             upload(ctx, ctx->chunk_order, L.run.data(), L.run.size(), "upload chunk_order") &&
              upload(ctx, ctx->rep_f, L.rep_f.data(), L.rep_f.size(), "upload rep_f") &&
              upload(ctx, ctx->rep_b, L.rep_b.data(), L.rep_b.size(), "upload rep_b") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask");
@@ -911,6 +902,13 @@ extern "C" int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info) {
     info->precision = ctx->precision;
     info->sweep_levels = ctx->nlev;
     info->device_bytes = ctx->device_bytes;
+    info->tile_cells = ctx->tile_cells;
+    info->sweep_waves = ctx->sweep_waves;
+    info->sweep_tiles = ctx->ntiles;
+    info->sweep_workgroups = ctx->sweep_by_tiles ? std::min(ctx->sweep_grid, ctx->ntiles)
+                                                 : std::min(ctx->sweep_grid, (ctx->nchunk + ctx->sweep_waves - 1) /
+                                                                                 ctx->sweep_waves);
+    info->sweep_by_tiles = ctx->sweep_by_tiles;
     return MFS_OK;
 }
 
@@ -966,6 +964,8 @@ struct mfs_batch {
 
 namespace {
 
+constexpr int kBatchSweepWaves = 8;  // waves per sweep workgroup of a batch
+
 template <typename R>
 size_t batch_entry_bytes() {
     return sizeof(dev::BatchSolve<R>);
@@ -987,11 +987,17 @@ bool batch_rebuild_maps(mfs_batch* b) {
         mfs_ctx* c = b->slots[b->active[a]];
         cs.insert(cs.end(), (size_t)c->nblk_cell, (int)a);
         ns.insert(ns.end(), (size_t)nblk_xcd(c->nN), (int)a);
-        for (int k = 0; k < c->ntiles; ++k) {
-            itf.push_back(make_int2((int)a, k));
-            itb.push_back(make_int2((int)a, c->ntiles - 1 - k));
-        }
     }
+    // flat schedule across solves: the u-th chunk (in level order) of every
+    // solve, u = 0, 1, ...; each solve's chunks stay in its sweep order
+    int maxch = 0;
+    for (size_t a = 0; a < b->active.size(); ++a) maxch = std::max(maxch, b->slots[b->active[a]]->nchunk);
+    for (int u = 0; u < maxch; ++u)
+        for (size_t a = 0; a < b->active.size(); ++a)
+            if (u < b->slots[b->active[a]]->nchunk) {
+                itf.push_back(make_int2((int)a, u));
+                itb.push_back(make_int2((int)a, u));
+            }
     b->ncell_blk = (int)cs.size();
     b->nnode_blk = (int)ns.size();
     b->nitems = (int)itf.size();
@@ -1063,17 +1069,13 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     E.fno = c->fno;
     E.fn = c->fn;
     E.btype = c->btype;
-    E.cnode = c->cnode;
-    E.cnperm = c->cnperm;
-    E.fb_cell = c->fb_cell;
-    E.fb_fo = c->fb_fo;
-    E.fb_face = c->fb_face;
-    E.fb_slot = c->fb_slot;
     auto& A = E.sw;
     A.nC = c->nC;
     A.nchunks = c->nchunk;
     A.ntiles = c->ntiles;
     A.tile_chunk = c->tile_chunk;
+    A.run = c->chunk_order;
+    A.flat = 1;
     A.chunk = c->chunk;
     A.cfs = c->cfs;
     A.fcell = c->fcell;
@@ -1166,16 +1168,13 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         return MFS_ERR_HIP;
     b->table_pending = true;
     auto* D = static_cast<dev::BatchSolve<R>*>(b->table_dev);
-    size_t lds_flux = 0;  // shared-face residual: 7 doubles per distinct face of the largest block
-    if (MAXF == 4)
-        for (int a = 0; a < ns; ++a)
-            lds_flux = std::max(lds_flux, sizeof(double) * 7 * (size_t)std::max(b->slots[b->active[a]]->fb_maxu, 1));
-    const dim3 sweep_grid((unsigned)std::min(b->sweep_grid, b->nitems));
+    const dim3 sweep_grid((unsigned)std::max(1, std::min(b->sweep_grid, (b->nitems + kBatchSweepWaves - 1) /
+                                                                         kBatchSweepWaves)));
     for (int i = 0; i < nsteps; ++i) {
         const dev::BatchSolve<R>* Tf = D + (size_t)i * 2 * ns;
         const dev::BatchSolve<R>* Tb = Tf + ns;
-        hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), lds_flux, b->stream,
-                           Tf, b->cell_solve);
+        hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
+                           b->cell_solve);
         hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(kBlock), 0, b->stream, Tf);
         hipLaunchKernelGGL((dev::k_b_clamp<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                            b->cell_solve);
@@ -1185,13 +1184,13 @@ int batch_step_t(mfs_batch* b, int nsteps) {
             void* args[] = {(void*)&Tf, (void*)&b->items_f, (void*)&n_it};
             if (!batch_ok(b,
                           hipLaunchCooperativeKernel((const void*)dev::k_b_sweep<true, LPC, R>, sweep_grid,
-                                                     dim3(dev::kSweepBlock), args, 0, b->stream),
+                                                     dim3(64 * kBatchSweepWaves), args, 0, b->stream),
                           "batch forward sweep"))
                 return MFS_ERR_HIP;
             void* argsb[] = {(void*)&Tb, (void*)&b->items_b, (void*)&n_it};
             if (!batch_ok(b,
                           hipLaunchCooperativeKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
-                                                     dim3(dev::kSweepBlock), argsb, 0, b->stream),
+                                                     dim3(64 * kBatchSweepWaves), argsb, 0, b->stream),
                           "batch backward sweep"))
                 return MFS_ERR_HIP;
         }
@@ -1220,9 +1219,9 @@ int batch_step_t(mfs_batch* b, int nsteps) {
 template <int LPC, typename R>
 int batch_sweep_occupancy(int* per_cu) {
     int a = 0, c = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_b_sweep<true, LPC, R>, dev::kSweepBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, dev::k_b_sweep<true, LPC, R>, 64 * kBatchSweepWaves, 0) !=
             hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, dev::k_b_sweep<false, LPC, R>, dev::kSweepBlock, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, dev::k_b_sweep<false, LPC, R>, 64 * kBatchSweepWaves, 0) !=
             hipSuccess)
         return -1;
     *per_cu = std::min(a, c);

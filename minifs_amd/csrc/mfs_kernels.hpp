@@ -346,136 +346,6 @@ __global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restric
                               diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
 }
 
-// The same residual, time step and Diag for tetrahedral meshes, with the
-// Roe flux of each face evaluated once per workgroup: workgroup b takes the
-// 256 cells with mesh ids 256b..256b+255 (a compact region in TDTree
-// numbering, where about two thirds of the cells' face slots are distinct
-// faces), evaluates the flux of each distinct face into LDS with the
-// pressures of both sides, then every cell adds its faces' fluxes in its
-// face order.  Inputs of each flux (q[c1], q[c2], the face normal and area)
-// and every sum are those of flux_ts_body, so the results are the same bits.
-// sh: 7 doubles per distinct face (flux, p of c1, p of c2).
-template <bool MIXED>
-__device__ __forceinline__ void flux_ts_shared_body(int b, const int* __restrict__ fb_cell, const int* __restrict__ fb_fo,
-                                                    const int* __restrict__ fb_face,
-                                                    const unsigned short* __restrict__ fb_slot,
-                                                    const int* __restrict__ cfo, const int* __restrict__ cf,
-                                                    const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
-                                                    const double2* __restrict__ fdn, const double* __restrict__ q,
-                                                    const double* __restrict__ vol, double* __restrict__ res,
-                                                    double* __restrict__ dt, double* __restrict__ diag,
-                                                    double* __restrict__ pblk, const FlowConst& K, double cfl,
-                                                    double cfl_min, float* __restrict__ q32,
-                                                    float* __restrict__ diag32, double* sh) {
-    const int f0 = fb_fo[b], nu = fb_fo[b + 1] - f0;
-    for (int k = (int)threadIdx.x; k < nu; k += kBlock) {
-        const int f = fb_face[f0 + k];
-        const int2 cc = fcell[f];
-        const double4 g = fgeo[f];
-        double ql[5], qr[5], F[5];
-        load5(q + (size_t)cc.x * 5, ql);
-        load5(q + (size_t)cc.y * 5, qr);
-        roe_flux<double>(ql, qr, g.x, g.y, g.z, g.w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
-#pragma unroll
-        for (int j = 0; j < 5; ++j) sh[k * 7 + j] = F[j];
-        sh[k * 7 + 5] = ql[4];
-        sh[k * 7 + 6] = qr[4];
-    }
-    __syncthreads();
-    const int c = fb_cell[(size_t)b * kBlock + threadIdx.x];
-    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double vmin = kBIGd, vmax = 0.0;
-    if (c >= 0) {
-        double qc[5];
-        load5(q + (size_t)c * 5, qc);
-        int fs[4];
-        cell_faces<4>(cfo, cf, c, fs);
-        int2 cc[4];
-        double4 g[4];
-        double2 d[4];
-        int sl[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            cc[s] = fcell[fs[s]];
-            g[s] = fgeo[fs[s]];
-            d[s] = fdn[fs[s]];
-            sl[s] = fb_slot[(size_t)c * 4 + s];
-        }
-        const double c2tmp = K.gam * (qc[4] + K.p_bar) / qc[0];
-        const double a = sqrt(c2tmp);
-        double dtc = kBIGd;
-        int det = 1;
-        double eig[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const bool first = (cc[s].x == c);
-            const double* e = sh + sl[s] * 7;
-            const double dn = first ? d[s].x : d[s].y;
-            double vn = qc[1] * g[s].x + qc[2] * g[s].y + qc[3] * g[s].z;
-            vn = fabs(vn);
-            eig[s] = vn + a;
-            const double pl = e[5], pr = e[6];
-            const double dp = fabs(pr - pl) / (pr + pl + K.p_bar + K.p_bar);
-            if (first) {
-#pragma unroll
-                for (int j = 0; j < 5; ++j) r[j] -= e[j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < 5; ++j) r[j] += e[j];
-            }
-            dtc = smin(dtc, dn / eig[s]);
-            if (dp > 0.1) det = 0;
-        }
-#pragma unroll
-        for (int j = 0; j < 5; ++j) res[(size_t)c * 5 + j] = r[j];
-        const double p = qc[4];
-        double cfl_tmp;
-        if (p > K.p_break)
-            cfl_tmp = cfl;
-        else if (p < K.p_min)
-            cfl_tmp = cfl_min;
-        else
-            cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
-        if (!det) cfl_tmp = cfl_min;
-        dtc *= cfl_tmp;
-        dt[c] = dtc;
-        vmin = dtc;
-        vmax = dtc;
-        double dg = vol[c] / dtc;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
-        diag[c] = dg;
-        if (MIXED) {
-            diag32[c] = (float)dg;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) q32[(size_t)c * 5 + j] = (float)qc[j];
-        }
-    }
-    double s2[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
-    block_sum<5>(s2, pblk);
-    block_minmax(vmin, vmax, pblk + 5, pblk + 6);
-}
-
-template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_flux_ts_shared(const int* __restrict__ fb_cell, const int* __restrict__ fb_fo,
-                                                           const int* __restrict__ fb_face,
-                                                           const unsigned short* __restrict__ fb_slot,
-                                                           const int* __restrict__ cfo, const int* __restrict__ cf,
-                                                           const int2* __restrict__ fcell,
-                                                           const double4* __restrict__ fgeo,
-                                                           const double2* __restrict__ fdn, const double* __restrict__ q,
-                                                           const double* __restrict__ vol, double* __restrict__ res,
-                                                           double* __restrict__ dt, double* __restrict__ diag,
-                                                           double* __restrict__ part, FlowConst K, double cfl,
-                                                           double cfl_min, float* __restrict__ q32,
-                                                           float* __restrict__ diag32) {
-    extern __shared__ double sh_faces[];
-    flux_ts_shared_body<MIXED>(xcd_block(), fb_cell, fb_fo, fb_face, fb_slot, cfo, cf, fcell, fgeo, fdn, q, vol, res, dt,
-                               diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32, sh_faces);
-}
-
 // Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
 // With `run`, only cells whose dt exceeds ratio_dtmax*dt_min are clamped
 // and recomputed (compute_timestep.cpp:929-945), and only when the
@@ -665,11 +535,6 @@ __global__ void k_node_symm(int nsym, const int* __restrict__ sym_node, const do
 // 1889-1897): face values from the node values (interior faces, SYMM and
 // INTERFACE boundary faces) or the face-centre average of cell and ghost;
 // outputs in the reference's per-variable blocks of nC+nB.
-template <typename R>
-__device__ __forceinline__ R pick4(const R (&nv)[4][5], int i, int j) {
-    return i == 0 ? nv[0][j] : (i == 1 ? nv[1][j] : (i == 2 ? nv[2][j] : nv[3][j]));
-}
-
 template <int MAXF, typename R>
 __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __restrict__ cfo,
                                           const int* __restrict__ cf, const int2* __restrict__ fcell,
@@ -677,8 +542,7 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
                                           const int* __restrict__ fno, const int* __restrict__ fn,
                                           const int* __restrict__ btype, const double* __restrict__ q,
                                           const R* __restrict__ qn, const double* __restrict__ vol,
-                                          R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz,
-                                          const int4* __restrict__ cnode, const int* __restrict__ cnperm) {
+                                          R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz) {
     if (c >= nC) return;
     R ax[5] = {R(0), R(0), R(0), R(0), R(0)}, ay[5] = {R(0), R(0), R(0), R(0), R(0)},
       az[5] = {R(0), R(0), R(0), R(0), R(0)};
@@ -686,23 +550,16 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
     const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
     int2 cc[MAXF];
     typename Real<R>::vec4 g[MAXF];
+    // tetrahedral meshes: every face is a triangle and its nodes sit at 3f
+    int tn[MAXF == 4 ? 12 : 1];
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
         cc[s] = fcell[fs[s]];
         g[s] = fgeo[fs[s]];
-    }
-    // tetrahedra: the cell's 4 node values, gathered once; face s takes
-    // three of them in its own node order (positions in cnperm)
-    R nv[MAXF == 4 ? 4 : 1][5];
-    int perm = 0;
-    if constexpr (MAXF == 4) {
-        const int4 cn = cnode[c];
-        perm = cnperm[c];
-        const int ids[4] = {cn.x, cn.y, cn.z, cn.w};
+        if constexpr (MAXF == 4) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int j = 0; j < 5; ++j) nv[a][j] = qn[(size_t)ids[a] * 5 + j];
+            for (int i = 0; i < 3; ++i) tn[3 * s + i] = fn[3 * fs[s] + i];
+        }
     }
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
@@ -718,13 +575,18 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
             R sum[5] = {R(0), R(0), R(0), R(0), R(0)};
             R cnt;
             if constexpr (MAXF == 4) {
-                const int p = perm >> (6 * s);
-                const int i0 = p & 3, i1 = (p >> 2) & 3, i2 = (p >> 4) & 3;
+                R v0[5], v1[5], v2[5];
 #pragma unroll
                 for (int j = 0; j < 5; ++j) {
-                    sum[j] += pick4<R>(nv, i0, j);
-                    sum[j] += pick4<R>(nv, i1, j);
-                    sum[j] += pick4<R>(nv, i2, j);
+                    v0[j] = qn[(size_t)tn[3 * s] * 5 + j];
+                    v1[j] = qn[(size_t)tn[3 * s + 1] * 5 + j];
+                    v2[j] = qn[(size_t)tn[3 * s + 2] * 5 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    sum[j] += v0[j];
+                    sum[j] += v1[j];
+                    sum[j] += v2[j];
                 }
                 cnt = R(3.0);
             } else {
@@ -774,8 +636,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __res
                                                  const int* __restrict__ fno, const int* __restrict__ fn,
                                                  const int* __restrict__ btype, const double* __restrict__ q,
                                                  const R* __restrict__ qn, const double* __restrict__ vol,
-                                                 R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz,
-                                                 const int4* __restrict__ cnode, const int* __restrict__ cnperm) {
+                                                 R* __restrict__ gx, R* __restrict__ gy, R* __restrict__ gz) {
     grad_body<MAXF, R>(xcd_block() * kBlock + (int)threadIdx.x, nC, n, cfo, cf, fcell, fgeo, fno, fn, btype, q, qn,
-                       vol, gx, gy, gz, cnode, cnperm);
+                       vol, gx, gy, gz);
 }

@@ -10,14 +10,13 @@
 
 namespace mfs {
 
-// Cells per storage tile of the device order (a power of two is not needed).
-constexpr int kTileCells = 4096;
 
 namespace {
 constexpr double kTINY_ = 1.e-40;
 }
 
-std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
+std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) {
+    if (kTileCells < 64) return "tile size must be at least 64 cells";
     if (v.nTCell <= 0 || v.nTFace <= 0 || v.nTNode <= 0 || v.nBFace < 0 || v.nBFace > v.nTFace)
         return "mesh view has invalid sizes";
     if (!v.x || !v.y || !v.z || !v.nNPF || !v.f2n || !v.f2c || !v.xfn || !v.yfn || !v.zfn || !v.area || !v.xfc ||
@@ -395,6 +394,12 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         }
         const int nchunk = (int)(L.chunk.size() / 2);
         L.tile_chunk.push_back(nchunk);
+        // flat schedule: every chunk of level l before any of level l+1
+        L.run.resize(nchunk);
+        std::iota(L.run.begin(), L.run.end(), 0);
+        std::stable_sort(L.run.begin(), L.run.end(), [&](int a, int b) {
+            return L.flevel[L.cell_of_dev[L.chunk[2 * a]]] < L.flevel[L.cell_of_dev[L.chunk[2 * b]]];
+        });
         L.rep_f.assign(nchunk, -1);
         L.rep_b.assign(nchunk, -1);
         // rank of a dependency: level first, then "in another tile"
@@ -430,84 +435,6 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L) {
         L.cfs.assign((size_t)nC * L.lpc, -1);
         for (int d = 0; d < nC; ++d)
             for (int k = L.cfo[d]; k < L.cfo[d + 1]; ++k) L.cfs[(size_t)d * L.lpc + (k - L.cfo[d])] = L.cf[k];
-    }
-
-    // ---- tetrahedral meshes: cell node lists for the gradient's face means
-    L.cnode.clear();
-    L.cnperm.clear();
-    if (L.max_faces == 4) {
-        bool tet = true;
-        L.cnode.assign(4 * (size_t)nC, -1);
-        L.cnperm.assign(nC, 0);
-        for (int d = 0; d < nC && tet; ++d) {
-            int* cn = &L.cnode[4 * (size_t)d];
-            int nn = 0, perm = 0;
-            for (int k = L.cfo[d]; k < L.cfo[d + 1] && tet; ++k) {
-                const int f = L.cf[k];
-                if (L.fno[f + 1] - L.fno[f] != 3) {
-                    tet = false;
-                    break;
-                }
-                for (int i = 0; i < 3; ++i) {
-                    const int node = L.fn[L.fno[f] + i];
-                    int pos = -1;
-                    for (int a = 0; a < nn; ++a)
-                        if (cn[a] == node) pos = a;
-                    if (pos < 0) {
-                        if (nn == 4) {
-                            tet = false;
-                            break;
-                        }
-                        pos = nn;
-                        cn[nn++] = node;
-                    }
-                    perm |= pos << (6 * (k - L.cfo[d]) + 2 * i);
-                }
-            }
-            if (nn != 4) tet = false;
-            L.cnperm[d] = perm;
-        }
-        if (!tet) {
-            L.cnode.clear();
-            L.cnperm.clear();
-        }
-    }
-
-    // ---- tetrahedral meshes: residual workgroups over mesh-order cell blocks
-    L.fb_cell.clear();
-    L.fb_fo.clear();
-    L.fb_face.clear();
-    L.fb_slot.clear();
-    L.fb_maxu = 0;
-    if (!L.cnode.empty()) {
-        const int B = 256;
-        const int nfb = (nC + B - 1) / B;
-        const int nblk = (nfb + 7) / 8 * 8;
-        L.fb_cell.assign((size_t)nblk * B, -1);
-        L.fb_fo.assign((size_t)nblk + 1, 0);
-        L.fb_slot.assign(4 * (size_t)nC, 0);
-        std::vector<int> stamp(nF, -1), idx(nF, 0);
-        for (int b = 0; b < nblk; ++b) {
-            int nu = 0;
-            for (int t = 0; t < B && b < nfb; ++t) {
-                const int m = b * B + t;
-                if (m >= nC) break;
-                const int d = dcell[m];
-                L.fb_cell[(size_t)b * B + t] = d;
-                for (int k = L.cfo[d]; k < L.cfo[d + 1]; ++k) {
-                    const int f = L.cf[k];
-                    if (stamp[f] != b) {
-                        stamp[f] = b;
-                        idx[f] = nu++;
-                        L.fb_face.push_back(f);
-                    }
-                    L.fb_slot[4 * (size_t)d + (k - L.cfo[d])] = (unsigned short)idx[f];
-                }
-            }
-            if (nu > 65535) return "residual workgroup with too many faces";
-            L.fb_maxu = std::max(L.fb_maxu, nu);
-            L.fb_fo[b + 1] = L.fb_fo[b] + nu;
-        }
     }
 
     // ---- state, device order (ghosts unchanged)

@@ -39,19 +39,6 @@ struct HostLayout {
     std::vector<double> fgeo;                  // 4*nF: xfn, yfn, zfn, area
     std::vector<double> dn1, dn2;              // nF: |(x_fc - x_cc) . n| for each side
     std::vector<int> fno, fn;                  // face->node CSR in device node ids
-    // tetrahedral meshes: the 4 nodes of each device cell (device node ids)
-    // and, per face slot s, 6 bits at 6s: the positions in that list of the
-    // face's 3 nodes in the face's own node order (empty otherwise)
-    std::vector<int> cnode;
-    std::vector<int> cnperm;
-    // tetrahedral meshes, residual workgroups: block b owns the cells with
-    // mesh ids 256b..256b+255 (fb_cell: device ids, -1 = none), the distinct
-    // faces they touch (fb_face[fb_fo[b]..fb_fo[b+1]), device ids) and, per
-    // device cell and face slot, that face's index in its block's list.
-    // Blocks are padded to a multiple of 8 (empty blocks).
-    std::vector<int> fb_cell, fb_fo, fb_face;
-    std::vector<unsigned short> fb_slot;
-    int fb_maxu = 0;
     std::vector<int> btype;                    // nB, by boundary face (ghost) index k
     // cells: device face ids in ascending mesh face id (CalC2F,
     // grid_polyhedra_auxiliary_functions.cpp:1345-1399)
@@ -76,13 +63,17 @@ struct HostLayout {
     int lpc = 4, cpc = 16;
     std::vector<int> chunk;                    // 2 per chunk (device order): first device cell, cell count
     std::vector<int> tile_chunk;               // ntile+1: first chunk of each storage tile
+    std::vector<int> run;                      // chunks by forward level (then tile): the flat schedule
     std::vector<int> rep_f, rep_b;             // per chunk: dependency expected to be published last (-1: none)
     std::vector<int> cfs;                      // lpc slots per device cell: device face id or -1
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
 };
 
+// Cells per storage tile of the device order (the sweeps' workgroup unit).
+constexpr int kDefaultTileCells = 4096;
+
 // Returns "" on success.
-std::string build_layout(const mfs_mesh_view& v, HostLayout& L);
+std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int tile_cells = kDefaultTileCells);
 
 }  // namespace mfs

@@ -16,17 +16,20 @@
 //
 // Schedule: cells are stored by tile (4096 consecutive mesh ids, a compact
 // region in TDTree numbering), then forward level; a chunk is up to 64/LPC
-// cells of one (tile, level) group.  Every forward dependency of a chunk
-// lies in an earlier chunk of the same tile or in an earlier tile, every
-// backward dependency in a later one.  A workgroup of W waves owns a tile:
-// wave w takes the tile's chunks w, w+W, ... in level order (reverse level
-// order for the backward sweep), so dependencies inside the tile are
-// published by waves of the same CU.  Tiles go to workgroups round-robin
-// over a cooperative (co-resident) grid, each workgroup walking its tiles
-// in sweep order and each wave finishing its chunks of one tile before the
-// next.  The first unfinished chunk in sweep order is then always being
-// worked on and its dependencies are finished, so the sweep cannot
-// deadlock.
+// cells of one (tile, level) group, so its cells never depend on one
+// another.  Two schedules, both on a cooperative (co-resident) grid:
+//   flat (default): the chunks in forward level order (`run`); chunk u goes
+//     to wave u mod (all waves of the grid), each wave walking its chunks
+//     in increasing u; the backward sweep walks the reverse order.
+//   tiles: a workgroup of W waves owns a tile, wave w taking the tile's
+//     chunks w, w+W, ... in level order; tiles go to workgroups
+//     round-robin, each workgroup walking its tiles in sweep order.
+// In both, every dependency of a chunk comes earlier in the sweep order and
+// each wave takes its chunks in that order, so the first unfinished chunk
+// is always being worked on with its dependencies finished: the sweep
+// cannot deadlock.  (On 6M cells the flat schedule measured 2x faster:
+// dependencies inside a tile cost the same device-coherent hop as across
+// tiles, and a workgroup held by one tile delays its next.)
 //
 // Publication: before a sweep every DQ slot it writes holds a NaN payload
 // (the sentinel) that arithmetic never produces.  A cell writes its 5 final
@@ -94,6 +97,8 @@ struct SweepArgs {
     int nC, nchunks;
     int ntiles;
     const int* tile_chunk;                  // ntiles+1: first chunk of each tile
+    const int* run;                         // chunks in forward level order (flat schedule)
+    int flat;                               // 1: flat schedule, 0: tiles
     const int2* chunk;                      // device order: first device cell, cell count
     const int* cfs;                         // LPC face slots per device cell (-1 = none)
     const int2* fcell;
@@ -293,6 +298,14 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(SweepArgs<R> A, FlowConst
     if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int W = (int)blockDim.x / 64;
     const int w = (int)threadIdx.x / 64;
+    if (A.flat) {
+        const int G = (int)gridDim.x * W;
+        for (int u = (int)blockIdx.x * W + w; u < A.nchunks; u += G) {
+            const int t = A.run[FWD ? u : A.nchunks - 1 - u];
+            if (!sweep_chunk<FWD, LPC, R>(A, K, t)) return;
+        }
+        return;
+    }
     for (int kk = blockIdx.x; kk < A.ntiles; kk += gridDim.x) {
         const int k = FWD ? kk : A.ntiles - 1 - kk;
         const int c0 = A.tile_chunk[k], c1 = A.tile_chunk[k + 1];
