@@ -260,6 +260,9 @@ def mode_converge(args) -> dict:
             s.step(k)
             done += k
             h = s.history()
+            if done % (20 * args.check_every) == 0:
+                print(f"[converge] {name}: {done} iterations, drop {h[-1, 0] / h[0, 0]:.3e}, "
+                      f"{time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
             if h[-1, 0] <= args.drop * h[0, 0]:
                 break
         s.sync()

@@ -222,8 +222,8 @@ int mfs_ctx_reset_timing(mfs_ctx* ctx);
 /* Batched multi-solve.  No counterpart in the reference, which runs one
  * solve per process (src/main.cpp:198-291).  A batch steps several
  * independent solves, each with its own mesh, flow condition and device
- * arrays, with one launch per kernel for all of them (the sweeps walk a
- * list of (solve, tile) items).  Each solve's arithmetic is the
+ * arrays, with one launch per kernel for all of them (the sweeps walk the
+ * solves' chunks interleaved, each solve's in its own level order).  Each solve's arithmetic is the
  * single-solve path's, so its results are the same alone and in a batch.
  * All solves of a batch share one precision and one cell family
  * (tetrahedral meshes, or meshes with up to 6 faces per cell).          */

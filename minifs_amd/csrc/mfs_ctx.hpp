@@ -88,7 +88,7 @@ struct mfs_ctx {
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;
     int nlev = 0;
-    int sweep_grid = 0;  // co-resident waves of the cooperative sweep launch
+    int sweep_grid = 0;  // resident workgroups of the sweep launch
     // reductions and history
     double* part = nullptr;  // nblk * 8: 5 squared-norm partials, dt min, dt max, spare
     double* run = nullptr;   // [0] dt_max [1] dt_min [2] clamp flag [3..7] last norms

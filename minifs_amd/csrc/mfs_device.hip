@@ -29,6 +29,24 @@ constexpr int kBlock = 256;
 constexpr int kXcd = 8;  // XCDs of an MI355X; cell-kernel grids are a multiple of this
 constexpr int kSweepWave = 64;
 constexpr int kSweepBlock = 1024;  // largest sweep workgroup (16 waves); one tile per workgroup
+// Resident sweep waves per CU.  The sweeps need every wave of the grid
+// resident at once, and the occupancy query is not a safe bound: for the
+// FP32 sweeps (72 VGPRs, 103-107 SGPRs) it reports 7 workgroups of 4 waves
+// per CU while an MI355X holds only 6 of them at once (the SGPRs are the
+// limit), and such a grid waits on chunks owned by waves that never start
+// until its bounded waits expire.  16 waves per CU (4 per SIMD) were all
+// resident for both sweep footprints, and more sweep waves do not run
+// faster.
+//
+// The sweeps are plain launches of that resident-sized grid.  They never
+// synchronise the whole grid, so a cooperative launch would add only its
+// size check, and it takes a device-wide wave-sync resource that one
+// process at a time can hold (a second process's cooperative launch
+// fails).  The context's stream runs the solver's kernels one after
+// another, so a sweep starts on an otherwise idle device unless another
+// process shares the GPU; if that keeps part of the grid from starting,
+// the bounded waits end the sweep with an error instead of a hang.
+constexpr int kSweepWavesPerCU = 16;
 constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wave
 
 #include "mfs_kernels.hpp"
@@ -43,6 +61,7 @@ namespace {
 
 using dev::kBlock;
 using dev::kSweepWave;
+using dev::kSweepWavesPerCU;
 
 bool hip_ok(mfs_ctx* ctx, hipError_t e, const char* what) {
     if (e == hipSuccess) return true;
@@ -262,7 +281,7 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
         A.rep = ctx->rep_f;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0,
+                    hipLaunchKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0,
                                                ctx->stream),
                     "k_sweep<fwd>"))
             return false;
@@ -272,7 +291,7 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
         A.rep = ctx->rep_b;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchCooperativeKernel((const void*)dev::k_sweep<false, LPC, R>, grid, block, args, 0,
+                    hipLaunchKernel((const void*)dev::k_sweep<false, LPC, R>, grid, block, args, 0,
                                                ctx->stream),
                     "k_sweep<bwd>"))
             return false;
@@ -517,7 +536,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
          dalloc(ctx, ctx->err, 4, "err");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
-        // co-resident waves for the cooperative sweep launch
+        // resident workgroups of the sweep launch
         int per_cu = 0, cus = 0;
         const bool mixed = ctx->precision == MFS_MIXED;
         const int blk = 64 * ctx->sweep_waves;
@@ -527,7 +546,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
         ok = occ == 0 &&
              hip_ok(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "CU count");
         if (!ok) set_error("mfs_ctx_create: occupancy query failed");
-        ctx->sweep_grid = std::max(1, per_cu * cus);  // co-resident workgroups
+        // co-resident workgroups
+        ctx->sweep_grid = std::max(1, std::min(per_cu, kSweepWavesPerCU / ctx->sweep_waves) * cus);
     }
     if (ok) {
         std::vector<int2> fc(L.nF);
@@ -1183,13 +1203,13 @@ int batch_step_t(mfs_batch* b, int nsteps) {
             int n_it = b->nitems;
             void* args[] = {(void*)&Tf, (void*)&b->items_f, (void*)&n_it};
             if (!batch_ok(b,
-                          hipLaunchCooperativeKernel((const void*)dev::k_b_sweep<true, LPC, R>, sweep_grid,
+                          hipLaunchKernel((const void*)dev::k_b_sweep<true, LPC, R>, sweep_grid,
                                                      dim3(64 * kBatchSweepWaves), args, 0, b->stream),
                           "batch forward sweep"))
                 return MFS_ERR_HIP;
             void* argsb[] = {(void*)&Tb, (void*)&b->items_b, (void*)&n_it};
             if (!batch_ok(b,
-                          hipLaunchCooperativeKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
+                          hipLaunchKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
                                                      dim3(64 * kBatchSweepWaves), argsb, 0, b->stream),
                           "batch backward sweep"))
                 return MFS_ERR_HIP;
@@ -1286,7 +1306,7 @@ extern "C" int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_
             set_error("mfs_batch_add: occupancy query failed");
             return MFS_ERR_HIP;
         }
-        b->sweep_grid = std::max(1, per_cu * cus);
+        b->sweep_grid = std::max(1, std::min(per_cu, kSweepWavesPerCU / kBatchSweepWaves) * cus);
     }
     // the solve's work goes to the batch stream from now on
     (void)hipStreamSynchronize(c->stream);

@@ -17,7 +17,7 @@
 // Schedule: cells are stored by tile (4096 consecutive mesh ids, a compact
 // region in TDTree numbering), then forward level; a chunk is up to 64/LPC
 // cells of one (tile, level) group, so its cells never depend on one
-// another.  Two schedules, both on a cooperative (co-resident) grid:
+// another.  Two schedules, both on a grid sized to be resident at once:
 //   flat (default): the chunks in forward level order (`run`); chunk u goes
 //     to wave u mod (all waves of the grid), each wave walking its chunks
 //     in increasing u; the backward sweep walks the reverse order.

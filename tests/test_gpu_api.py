@@ -120,3 +120,33 @@ def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles)
     np.testing.assert_allclose(a.history(), b.history(), rtol=1e-12, atol=0)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("entry_points", [False, True])
+def test_native_driver_matches_python_api(entry_points):
+    """mfs_solve (the native driver of the reference's main loop) gives the
+    same residual history record as the library driven from Python, through
+    mfs_step or through the kernel entry points."""
+    import json
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(mfs.__file__), "mfs_solve")
+    cmd = [exe, "--box", "8", "--steps", "4", "--warmup", "1", "--mach", "0.8", "--alpha", "1.25", "--cfl", "10",
+           "--json"] + (["--entry-points"] if entry_points else [])
+    print("[native]", " ".join(cmd), flush=True)
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
+    g = mfs.Grid.box_bump(8).decouple().preprocess(p)
+    s = mfs.Solver(g, p)
+    s.step(5)
+    h = s.history()
+    s.close()
+    assert rec["cells"] == g.info["nTCell"]
+    got = np.array(rec["last_residual"] + [rec["dt_max"], rec["dt_min"]])
+    if entry_points:  # the separate flux kernel sums the norm in other blocks
+        np.testing.assert_allclose(got, h[-1], rtol=1e-12, atol=0)
+    else:
+        np.testing.assert_array_equal(got, h[-1])

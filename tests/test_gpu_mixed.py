@@ -58,3 +58,25 @@ def test_mixed_converges_to_the_fp64_solution(mach, alpha):
     assert abs(L32 - L64) <= 1e-9 * abs(L64)
     # the FP32 operator does not slow the convergence much
     assert len(h32) <= 1.5 * len(h64) + 100
+
+
+@pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
+@pytest.mark.parametrize("waves", [4, 8, 16])
+def test_full_sweep_grid_is_resident(precision, waves):
+    """A mesh with more sweep chunks than the sweep grid has waves, so every
+    workgroup of the cooperative grid is launched and must be resident for
+    the sweeps to finish: at most 16 waves per CU, for every workgroup size
+    and both operator precisions.  The mixed run's first residual is the
+    FP64 one."""
+    print(f"[mixed] full sweep grid precision={precision} waves={waves}", flush=True)
+    p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
+    g = mfs.Grid.box_bump(25).decouple().preprocess(p)
+    s = mfs.Solver(g, p, precision=precision, sweep_waves=waves)
+    info = s.info()
+    assert info["sweep_workgroups"] * waves <= 16 * 256
+    assert g.info["nTCell"] // 16 > info["sweep_workgroups"] * waves  # more chunks than waves
+    s.step(3)
+    h = s.history()
+    assert np.all(np.isfinite(h)) and h[-1, 0] < h[0, 0] * 10
+    s.close()
+    g.close()

@@ -245,6 +245,27 @@ def test_solver_without_gpu_fails_loudly():
         mfs.Solver(g, mfs.make_params(0.5, 0.0, 10.0))
 
 
+def test_native_driver_without_gpu_fails_loudly(tmp_path):
+    """The native driver (mfs_solve) builds the grid on the host, then stops
+    with an error at the device context when there is no GPU; it also
+    rejects bad arguments and reads a saved FlowStar mesh."""
+    from tests.conftest import gpu_available
+
+    exe = os.path.join(ROOT, "minifs_amd", "mfs_solve")
+    assert os.path.exists(exe)
+    bad = subprocess.run([exe, "--precision", "fp16"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 2 and "usage" in bad.stderr
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    out = subprocess.run([exe, "--box", "3", "--steps", "1"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "device context" in out.stderr
+    g = mfs.Grid.box_bump(2)
+    mesh, bc = str(tmp_path / "b.mfl"), str(tmp_path / "b.bc")
+    g.save(mesh, bc)
+    out = subprocess.run([exe, "--mesh", mesh, "--bc", bc], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "device context" in out.stderr
+
+
 def test_preprocess_rejects_unknown_patch(tmp_path):
     g = mfs.Grid.box_bump(2)
     mesh, bc = str(tmp_path / "a.mfl"), str(tmp_path / "a.bc")
