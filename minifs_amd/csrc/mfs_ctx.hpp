@@ -77,14 +77,16 @@ struct mfs_ctx {
     std::vector<int> node_of_dev;  // host: device node -> mesh node
     std::vector<int> cell_of_dev;  // host: device cell -> mesh cell
     // sweeps
-    int2* chunk = nullptr;  // per chunk (device order): first device cell, cell count
+    int4* crun = nullptr;       // per position of the flat order: first cell, count, rep fwd, rep bwd
+    int* pos = nullptr;         // per chunk (device order): its position in the flat order
     int* tile_chunk = nullptr;  // ntiles+1: first chunk of each tile
-    int* chunk_order = nullptr; // chunks in forward level order (flat schedule)
+    int* snb = nullptr;         // lpc per device cell: neighbour across the slot's face
+    double4* sgeo = nullptr;    // lpc per device cell: normal away from the cell, area
+    float4* sgeo32 = nullptr;   // the same in FP32 (mixed precision)
     int sweep_by_tiles = 0;
     int ntiles = 0;
-    int *rep_f = nullptr, *rep_b = nullptr;
-    int* cfs = nullptr;     // lpc face slots per device cell
     int* depmask = nullptr;
+    int* tickets = nullptr;  // ticket counters of the flat sweep schedule: forward, then backward
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;
     int nlev = 0;

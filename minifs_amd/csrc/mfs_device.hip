@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "mfs_ctx.hpp"
 #include "mfs_mesh.hpp"
@@ -48,6 +49,14 @@ constexpr int kSweepBlock = 1024;  // largest sweep workgroup (16 waves); one ti
 // the bounded waits end the sweep with an error instead of a hang.
 constexpr int kSweepWavesPerCU = 16;
 constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wave
+// Ticket groups of the flat sweep schedule.  With n groups in use (64, or
+// fewer when the grid has fewer workgroups), position u of the flat order
+// belongs to group u mod n, and the waves of the workgroups b with
+// b mod n = g take group g's positions in order from the group's counter.  One counter for the whole grid would serialise every chunk on
+// one address (about 90 M atomics/s measured on an MI355X, 4.7 ms per
+// sweep of 6M cells); 64 counters cost well under a tenth of a sweep.
+constexpr int kTicketGroups = 64;
+constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
 
 #include "mfs_kernels.hpp"
 #include "mfs_sweep.hpp"
@@ -62,6 +71,8 @@ namespace {
 using dev::kBlock;
 using dev::kSweepWave;
 using dev::kSweepWavesPerCU;
+using dev::kTicketGroups;
+using dev::kTicketStride;
 
 bool hip_ok(mfs_ctx* ctx, hipError_t e, const char* what) {
     if (e == hipSuccess) return true;
@@ -249,19 +260,18 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     A.nchunks = ctx->nchunk;
     A.ntiles = ctx->ntiles;
     A.tile_chunk = ctx->tile_chunk;
-    A.run = ctx->chunk_order;
+    A.pos = ctx->pos;
     A.flat = ctx->sweep_by_tiles ? 0 : 1;
-    A.chunk = ctx->chunk;
-    A.cfs = ctx->cfs;
-    A.fcell = ctx->fcell;
+    A.crun = ctx->crun;
+    A.snb = ctx->snb;
     if constexpr (sizeof(R) == sizeof(double)) {
-        A.fgeo = ctx->fgeo;
+        A.sgeo = ctx->sgeo;
         A.q = ctx->q[ctx->cur];
         A.diag = ctx->diag;
         A.dqf = ctx->dqf;
         A.dqb = ctx->dq;
     } else {
-        A.fgeo = ctx->fgeo32;
+        A.sgeo = ctx->sgeo32;
         A.q = ctx->q32;
         A.diag = ctx->diag32;
         A.dqf = ctx->dqf32;
@@ -276,9 +286,13 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     const dim3 block(64 * ctx->sweep_waves);
     const int units = A.flat ? (A.nchunks + ctx->sweep_waves - 1) / ctx->sweep_waves : A.ntiles;
     const dim3 grid((unsigned)std::max(1, std::min(ctx->sweep_grid, units)));
+    A.ngroups = std::min(kTicketGroups, (int)grid.x);
+    int* const tk_f = ctx->tickets;
+    int* const tk_b = ctx->tickets + kTicketGroups * kTicketStride;
     {
         Timed t(ctx, "fwd_sweep");
-        A.rep = ctx->rep_f;
+        A.tickets = tk_f;
+        A.tickets_next = tk_b;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
                     hipLaunchKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0,
@@ -288,7 +302,8 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     }
     {
         Timed t(ctx, "bwd_sweep");
-        A.rep = ctx->rep_b;
+        A.tickets = tk_b;
+        A.tickets_next = tk_f;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
                     hipLaunchKernel((const void*)dev::k_sweep<false, LPC, R>, grid, block, args, 0,
@@ -520,20 +535,23 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     ctx->nsym = (int)L.sym_node.size();
     ok = ok && dalloc(ctx, ctx->sym_node, L.sym_node.size(), "sym_node") &&
          dalloc(ctx, ctx->sym_n, L.sym_n.size(), "sym_n");
-    ok = ok && dalloc(ctx, ctx->chunk, (size_t)ctx->nchunk, "chunk") && dalloc(ctx, ctx->cfs, L.cfs.size(), "cfs") &&
-         dalloc(ctx, ctx->tile_chunk, L.tile_chunk.size(), "tile_chunk") && dalloc(ctx, ctx->chunk_order, L.run.size(), "chunk_order") &&
-         dalloc(ctx, ctx->rep_f, L.rep_f.size(), "rep_f") &&
-         dalloc(ctx, ctx->rep_b, L.rep_b.size(), "rep_b") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
+    ok = ok && dalloc(ctx, ctx->crun, (size_t)ctx->nchunk, "chunk records") &&
+         dalloc(ctx, ctx->pos, (size_t)ctx->nchunk, "chunk positions") &&
+         dalloc(ctx, ctx->tile_chunk, L.tile_chunk.size(), "tile_chunk") &&
+         dalloc(ctx, ctx->snb, L.snb.size(), "slot neighbours") &&
+         dalloc(ctx, ctx->sgeo, L.snb.size(), "slot geometry") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
     const bool mixed = precision == MFS_MIXED;
     if (mixed) {
         ok = ok && dalloc(ctx, ctx->q32, 5 * n, "q32") && dalloc(ctx, ctx->diag32, L.nC, "diag32") &&
              dalloc(ctx, ctx->dqf32, 5 * (size_t)L.nC, "dq_forward32") && dalloc(ctx, ctx->dq32, 5 * n, "dq32") &&
              dalloc(ctx, ctx->gx32, 5 * n, "dqdx32") && dalloc(ctx, ctx->gy32, 5 * n, "dqdy32") &&
              dalloc(ctx, ctx->gz32, 5 * n, "dqdz32") && dalloc(ctx, ctx->qn32, 5 * (size_t)L.nN, "qnode32") &&
-             dalloc(ctx, ctx->ell_w32, ell_w.size(), "ell_w32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32");
+             dalloc(ctx, ctx->ell_w32, ell_w.size(), "ell_w32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32") &&
+             dalloc(ctx, ctx->sgeo32, L.snb.size(), "slot geometry (FP32)");
     }
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
-         dalloc(ctx, ctx->err, 4, "err");
+         dalloc(ctx, ctx->err, 4, "err") &&
+         dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
         // resident workgroups of the sweep launch
@@ -573,17 +591,16 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              upload(ctx, ctx->nden, L.nden.data(), L.nN, "upload nden") &&
              upload(ctx, ctx->sym_node, L.sym_node.data(), L.sym_node.size(), "upload sym") &&
              upload(ctx, ctx->sym_n, L.sym_n.data(), L.sym_n.size(), "upload sym_n") &&
-             upload(ctx, ctx->chunk, reinterpret_cast<const int2*>(L.chunk.data()), (size_t)ctx->nchunk, "upload chunk") &&
-             upload(ctx, ctx->cfs, L.cfs.data(), L.cfs.size(), "upload cfs") &&
+             upload(ctx, ctx->crun, reinterpret_cast<const int4*>(L.crun.data()), (size_t)ctx->nchunk, "upload crun") &&
+             upload(ctx, ctx->pos, L.pos.data(), L.pos.size(), "upload pos") &&
              upload(ctx, ctx->tile_chunk, L.tile_chunk.data(), L.tile_chunk.size(), "upload tile_chunk") &&
-             upload(ctx, ctx->chunk_order, L.run.data(), L.run.size(), "upload chunk_order") &&
-             upload(ctx, ctx->rep_f, L.rep_f.data(), L.rep_f.size(), "upload rep_f") &&
-             upload(ctx, ctx->rep_b, L.rep_b.data(), L.rep_b.size(), "upload rep_b") &&
+             upload(ctx, ctx->snb, L.snb.data(), L.snb.size(), "upload snb") &&
+             upload(ctx, ctx->sgeo, reinterpret_cast<const double4*>(L.sgeo.data()), L.snb.size(), "upload sgeo") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask");
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
         std::vector<float> f32;
-        std::vector<float4> g32;
+        std::vector<float4> g32, s32;
         if (ok && mixed) {
             // FP32 copies of the operator's static inputs; the ghost part of
             // the state snapshot is fixed (ghosts are frozen after set-up)
@@ -593,7 +610,12 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
                                      (float)L.fgeo[4 * (size_t)i + 2], (float)L.fgeo[4 * (size_t)i + 3]);
             f32.resize(std::max(ell_w.size(), 5 * n));
             for (size_t k = 0; k < ell_w.size(); ++k) f32[k] = (float)ell_w[k];
+            s32.resize(L.snb.size());
+            for (size_t i = 0; i < L.snb.size(); ++i)
+                s32[i] = make_float4((float)L.sgeo[4 * i], (float)L.sgeo[4 * i + 1], (float)L.sgeo[4 * i + 2],
+                                     (float)L.sgeo[4 * i + 3]);
             ok = upload(ctx, ctx->fgeo32, g32.data(), L.nF, "upload fgeo32") &&
+                 upload(ctx, ctx->sgeo32, s32.data(), s32.size(), "upload sgeo32") &&
                  upload(ctx, ctx->ell_w32, f32.data(), ell_w.size(), "upload ell_w32");
             for (size_t k = 0; k < 5 * n; ++k) f32[k] = (float)qa[k];
             ok = ok && upload(ctx, ctx->q32, f32.data(), 5 * n, "upload q32");
@@ -620,6 +642,9 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
             }
         }
         ok = ok && hip_ok(ctx, hipMemsetAsync(ctx->err, 0, sizeof(int) * 4, ctx->stream), "memset err") &&
+             hip_ok(ctx,
+                    hipMemsetAsync(ctx->tickets, 0, sizeof(int) * 2 * kTicketGroups * kTicketStride, ctx->stream),
+                    "memset tickets") &&
              hip_ok(ctx, hipMemsetAsync(ctx->dq, 0, sizeof(double) * 5 * n, ctx->stream), "memset dq") &&
              hip_ok(ctx, hipMemsetAsync(ctx->res, 0, sizeof(double) * 5 * L.nC, ctx->stream), "memset res") &&
              hip_ok(ctx, hipMemsetAsync(ctx->dt, 0, sizeof(double) * L.nC, ctx->stream), "memset dt") &&
@@ -1094,18 +1119,18 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     A.nchunks = c->nchunk;
     A.ntiles = c->ntiles;
     A.tile_chunk = c->tile_chunk;
-    A.run = c->chunk_order;
+    A.pos = c->pos;
     A.flat = 1;
-    A.chunk = c->chunk;
-    A.cfs = c->cfs;
-    A.fcell = c->fcell;
+    A.crun = c->crun;
+    A.snb = c->snb;
     A.q64 = c->q[cur];
     A.res = c->res;
     A.qout = c->q[1 - cur];
-    A.rep = nullptr;  // set per sweep direction by the kernel launch below
     A.depmask = c->depmask;
     A.cell0 = c->cell0;
     A.err = c->err;
+    A.tickets = A.tickets_next = nullptr;  // the batch sweeps assign (solve, position) items instead
+    A.ngroups = 0;
     if constexpr (sizeof(R) == sizeof(double)) {
         E.ell_w = c->ell_w;
         E.qn = c->qn;
@@ -1113,7 +1138,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
         E.gx = c->gx;
         E.gy = c->gy;
         E.gz = c->gz;
-        A.fgeo = c->fgeo;
+        A.sgeo = c->sgeo;
         A.q = c->q[cur];
         A.diag = c->diag;
         A.dqf = c->dqf;
@@ -1125,7 +1150,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
         E.gx = c->gx32;
         E.gy = c->gy32;
         E.gz = c->gz32;
-        A.fgeo = c->fgeo32;
+        A.sgeo = c->sgeo32;
         A.q = c->q32;
         A.diag = c->diag32;
         A.dqf = c->dqf32;
@@ -1174,10 +1199,8 @@ int batch_step_t(mfs_batch* b, int nsteps) {
             const int cur = c->cur ^ (i & 1);
             dev::BatchSolve<R>& Ef = H[(size_t)i * 2 * ns + a];
             fill_entry<R>(Ef, c, cur, blk_c, blk_n, c->iter_done + i, c->hist_n + i);
-            Ef.sw.rep = c->rep_f;
             dev::BatchSolve<R>& Eb = H[(size_t)i * 2 * ns + ns + a];
             Eb = Ef;
-            Eb.sw.rep = c->rep_b;
             blk_c += c->nblk_cell;
             blk_n += (int)nblk_xcd(c->nN);
         }

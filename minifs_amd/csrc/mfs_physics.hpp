@@ -105,42 +105,67 @@ MFS_HD void roe_flux(const R ql[5], const R qr[5], R ax, R ay, R az, R area, R g
 // FluxLUSGS3D (src/Kernels/forward_lusgs.cpp:1271-1324): neighbour state q,
 // neighbour increment DQ, normal n pointing from the swept cell to the
 // neighbour.
+// The part of FluxLUSGS3D that does not depend on DQ (the neighbour's
+// state, its flux and the spectral radius), so that a sweep can evaluate
+// it before the neighbour's DQ arrives.  flux_lusgs_pre followed by
+// flux_lusgs_post performs the reference's operations in its order.
 template <typename R>
-MFS_HD void flux_lusgs(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga, R flux[5]) {
-    R Q[5], rv2, v_n, p, peff, c2, eig, gam1 = gam - R(1.);
-    Q[0] = q[0];
-    Q[1] = q[0] * q[1];
-    Q[2] = q[0] * q[2];
-    Q[3] = q[0] * q[3];
+struct FluxPre {
+    R Q[5];          // conservative state of the neighbour
+    R rho, v_n, eig;  // density, normal velocity, lhs_omga * spectral radius
+    R peff, gam1;
+    R f[5];          // -F(Q) (components 1-4)
+};
+
+template <typename R>
+MFS_HD void flux_lusgs_pre(FluxPre<R>& P, const R q[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga) {
+    R rv2, p, c2;
+    P.gam1 = gam - R(1.);
+    P.Q[0] = q[0];
+    P.Q[1] = q[0] * q[1];
+    P.Q[2] = q[0] * q[2];
+    P.Q[3] = q[0] * q[3];
     rv2 = R(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     p = q[4];
-    Q[4] = p / gam1 + rv2;
+    P.Q[4] = p / P.gam1 + rv2;
 
-    v_n = q[1] * nx + q[2] * ny + q[3] * nz;
+    P.v_n = q[1] * nx + q[2] * ny + q[3] * nz;
     c2 = gam * (p + p_bar) / q[0];
-    eig = fabs(v_n) + sqrt(c2);
-    eig *= lhs_omga;
+    P.eig = fabs(P.v_n) + sqrt(c2);
+    P.eig *= lhs_omga;
 
-    peff = gam * p_bar / gam1;
-    flux[1] = -Q[1] * v_n - p * nx;
-    flux[2] = -Q[2] * v_n - p * ny;
-    flux[3] = -Q[3] * v_n - p * nz;
-    flux[4] = -(Q[4] + p + peff) * v_n;
+    P.peff = gam * p_bar / P.gam1;
+    P.f[1] = -P.Q[1] * P.v_n - p * nx;
+    P.f[2] = -P.Q[2] * P.v_n - p * ny;
+    P.f[3] = -P.Q[3] * P.v_n - p * nz;
+    P.f[4] = -(P.Q[4] + p + P.peff) * P.v_n;
+    P.rho = q[0];
+}
 
-    for (int i = 0; i < 5; i++) Q[i] += DQ[i];
+template <typename R>
+MFS_HD void flux_lusgs_post(const FluxPre<R>& P, const R DQ[5], R nx, R ny, R nz, R flux[5]) {
+    R Q[5], rv2, p, v_n;
+    for (int i = 0; i < 5; i++) Q[i] = P.Q[i] + DQ[i];
     rv2 = R(0.5) * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) / Q[0];
-    p = gam1 * (Q[4] - rv2);
+    p = P.gam1 * (Q[4] - rv2);
 
     flux[0] = DQ[1] * nx + DQ[2] * ny + DQ[3] * nz;
-    v_n *= q[0];
+    v_n = P.v_n * P.rho;
     v_n += flux[0];
     v_n /= Q[0];
-    flux[1] += Q[1] * v_n + p * nx;
-    flux[2] += Q[2] * v_n + p * ny;
-    flux[3] += Q[3] * v_n + p * nz;
-    flux[4] += (Q[4] + p + peff) * v_n;
+    flux[1] = P.f[1] + (Q[1] * v_n + p * nx);
+    flux[2] = P.f[2] + (Q[2] * v_n + p * ny);
+    flux[3] = P.f[3] + (Q[3] * v_n + p * nz);
+    flux[4] = P.f[4] + (Q[4] + p + P.peff) * v_n;
 
-    for (int i = 0; i < 5; i++) flux[i] -= eig * DQ[i];
+    for (int i = 0; i < 5; i++) flux[i] -= P.eig * DQ[i];
+}
+
+template <typename R>
+MFS_HD void flux_lusgs(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga, R flux[5]) {
+    FluxPre<R> P;
+    flux_lusgs_pre(P, q, nx, ny, nz, gam, p_bar, lhs_omga);
+    flux_lusgs_post(P, DQ, nx, ny, nz, flux);
 }
 
 // FluxLUSGS3D in increment form, for the FP32 operator of mixed precision.
@@ -155,29 +180,50 @@ MFS_HD void flux_lusgs(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p
 //   dF0 = DQ1..3.n,  dF_i = Q_i*dvn + DQ_i*v_n' + dp*n_i,  dF4 = (Q4+p+peff)*dvn + (DQ4+dp)*v_n'
 //   flux = dF - eig*DQ
 template <typename R>
-MFS_HD void flux_lusgs_delta(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga, R flux[5]) {
-    const R gam1 = gam - R(1.);
+struct FluxDeltaPre {
+    R rho, Q1, Q2, Q3, vv, v_n, eig, gam1;
+    R H;  // the energy flux factor Q4 + p + peff
+};
+
+template <typename R>
+MFS_HD void flux_lusgs_delta_pre(FluxDeltaPre<R>& P, const R q[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga) {
+    P.gam1 = gam - R(1.);
     const R rho = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
-    const R Q1 = rho * u, Q2 = rho * v, Q3 = rho * w;
-    const R vv = u * u + v * v + w * w;
-    const R Q4 = p / gam1 + R(0.5) * rho * vv;
-    const R v_n = u * nx + v * ny + w * nz;
-    const R eig = (fabs(v_n) + sqrt(gam * (p + p_bar) / rho)) * lhs_omga;
-    const R peff = gam * p_bar / gam1;
-    const R Q0n = rho + DQ[0];
+    P.rho = rho;
+    P.Q1 = rho * u;
+    P.Q2 = rho * v;
+    P.Q3 = rho * w;
+    P.vv = u * u + v * v + w * w;
+    const R Q4 = p / P.gam1 + R(0.5) * rho * P.vv;
+    P.v_n = u * nx + v * ny + w * nz;
+    P.eig = (fabs(P.v_n) + sqrt(gam * (p + p_bar) / rho)) * lhs_omga;
+    const R peff = gam * p_bar / P.gam1;
+    P.H = Q4 + p + peff;
+}
+
+template <typename R>
+MFS_HD void flux_lusgs_delta_post(const FluxDeltaPre<R>& P, const R DQ[5], R nx, R ny, R nz, R flux[5]) {
+    const R Q0n = P.rho + DQ[0];
     const R dmn = DQ[1] * nx + DQ[2] * ny + DQ[3] * nz;  // increment of the mass flux
-    const R dvn = (dmn - v_n * DQ[0]) / Q0n;
-    const R vn1 = v_n + dvn;
-    const R qd = Q1 * DQ[1] + Q2 * DQ[2] + Q3 * DQ[3];
+    const R dvn = (dmn - P.v_n * DQ[0]) / Q0n;
+    const R vn1 = P.v_n + dvn;
+    const R qd = P.Q1 * DQ[1] + P.Q2 * DQ[2] + P.Q3 * DQ[3];
     const R dd = DQ[1] * DQ[1] + DQ[2] * DQ[2] + DQ[3] * DQ[3];
-    const R drv2 = R(0.5) * (R(2.) * qd + dd - rho * vv * DQ[0]) / Q0n;
-    const R dp = gam1 * (DQ[4] - drv2);
+    const R drv2 = R(0.5) * (R(2.) * qd + dd - P.rho * P.vv * DQ[0]) / Q0n;
+    const R dp = P.gam1 * (DQ[4] - drv2);
     flux[0] = dmn;
-    flux[1] = Q1 * dvn + DQ[1] * vn1 + dp * nx;
-    flux[2] = Q2 * dvn + DQ[2] * vn1 + dp * ny;
-    flux[3] = Q3 * dvn + DQ[3] * vn1 + dp * nz;
-    flux[4] = (Q4 + p + peff) * dvn + (DQ[4] + dp) * vn1;
-    for (int i = 0; i < 5; i++) flux[i] -= eig * DQ[i];
+    flux[1] = P.Q1 * dvn + DQ[1] * vn1 + dp * nx;
+    flux[2] = P.Q2 * dvn + DQ[2] * vn1 + dp * ny;
+    flux[3] = P.Q3 * dvn + DQ[3] * vn1 + dp * nz;
+    flux[4] = P.H * dvn + (DQ[4] + dp) * vn1;
+    for (int i = 0; i < 5; i++) flux[i] -= P.eig * DQ[i];
+}
+
+template <typename R>
+MFS_HD void flux_lusgs_delta(const R q[5], const R DQ[5], R nx, R ny, R nz, R gam, R p_bar, R lhs_omga, R flux[5]) {
+    FluxDeltaPre<R> P;
+    flux_lusgs_delta_pre(P, q, nx, ny, nz, gam, p_bar, lhs_omga);
+    flux_lusgs_delta_post(P, DQ, nx, ny, nz, flux);
 }
 
 // The three-stage DQ positivity limiter (DQ_limit == 2) of both sweeps

@@ -435,6 +435,34 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
         L.cfs.assign((size_t)nC * L.lpc, -1);
         for (int d = 0; d < nC; ++d)
             for (int k = L.cfo[d]; k < L.cfo[d + 1]; ++k) L.cfs[(size_t)d * L.lpc + (k - L.cfo[d])] = L.cf[k];
+        // the sweeps' tables: chunk records by position, slot neighbours and
+        // oriented geometry (the normal is negated where the cell is the
+        // face's second cell, exactly as the sweep would negate it)
+        L.crun.resize(4 * (size_t)nchunk);
+        L.pos.resize(nchunk);
+        for (int u = 0; u < nchunk; ++u) {
+            const int t = L.run[u];
+            L.pos[t] = u;
+            L.crun[4 * (size_t)u] = L.chunk[2 * t];
+            L.crun[4 * (size_t)u + 1] = L.chunk[2 * t + 1];
+            L.crun[4 * (size_t)u + 2] = L.rep_f[t];
+            L.crun[4 * (size_t)u + 3] = L.rep_b[t];
+        }
+        L.snb.resize((size_t)nC * L.lpc);
+        L.sgeo.assign(4 * (size_t)nC * L.lpc, 0.0);
+        for (int d = 0; d < nC; ++d)
+            for (int sl = 0; sl < L.lpc; ++sl) {
+                const size_t i = (size_t)d * L.lpc + sl;
+                const int f = L.cfs[i];
+                if (f < 0) {
+                    L.snb[i] = d;
+                    continue;
+                }
+                const bool flip = (L.c2[f] == d);
+                L.snb[i] = flip ? L.c1[f] : L.c2[f];
+                for (int j = 0; j < 3; ++j) L.sgeo[4 * i + j] = flip ? -L.fgeo[4 * (size_t)f + j] : L.fgeo[4 * (size_t)f + j];
+                L.sgeo[4 * i + 3] = L.fgeo[4 * (size_t)f + 3];
+            }
     }
 
     // ---- state, device order (ghosts unchanged)

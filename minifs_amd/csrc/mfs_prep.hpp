@@ -66,6 +66,16 @@ struct HostLayout {
     std::vector<int> run;                      // chunks by forward level (then tile): the flat schedule
     std::vector<int> rep_f, rep_b;             // per chunk: dependency expected to be published last (-1: none)
     std::vector<int> cfs;                      // lpc slots per device cell: device face id or -1
+    // what the sweeps read, laid out so that a chunk needs no chain of
+    // lookups: per position u of `run`, {first device cell, cell count,
+    // rep_f, rep_b} of chunk run[u]; per chunk, its position; per device
+    // cell and slot, the neighbour across the slot's face (the cell itself
+    // when the slot is empty) and the face's unit normal pointing away from
+    // the cell with its area (zeros when empty)
+    std::vector<int> crun;                     // 4 per position
+    std::vector<int> pos;                      // per chunk
+    std::vector<int> snb;                      // lpc per device cell
+    std::vector<double> sgeo;                  // 4*lpc per device cell
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
 };

@@ -18,18 +18,22 @@
 // region in TDTree numbering), then forward level; a chunk is up to 64/LPC
 // cells of one (tile, level) group, so its cells never depend on one
 // another.  Two schedules, both on a grid sized to be resident at once:
-//   flat (default): the chunks in forward level order (`run`); chunk u goes
-//     to wave u mod (all waves of the grid), each wave walking its chunks
-//     in increasing u; the backward sweep walks the reverse order.
+//   flat (default): the chunks in forward level order (`run`), the backward
+//     sweep walking it from the end; position u belongs to ticket group
+//     u mod ngroups, and a free wave of the group takes the group's next
+//     position from the group's counter, so waves that finish early take
+//     more chunks and no wave's delay holds back the chunks after it.
 //   tiles: a workgroup of W waves owns a tile, wave w taking the tile's
 //     chunks w, w+W, ... in level order; tiles go to workgroups
 //     round-robin, each workgroup walking its tiles in sweep order.
 // In both, every dependency of a chunk comes earlier in the sweep order and
 // each wave takes its chunks in that order, so the first unfinished chunk
-// is always being worked on with its dependencies finished: the sweep
-// cannot deadlock.  (On 6M cells the flat schedule measured 2x faster:
-// dependencies inside a tile cost the same device-coherent hop as across
-// tiles, and a workgroup held by one tile delays its next.)
+// is always being worked on with its dependencies finished, or is the next
+// ticket of a group whose waves are all free: the sweep cannot deadlock
+// while every wave of the grid is resident.  (On 6M cells the flat
+// schedule measured 2x faster: dependencies inside a tile cost the same
+// device-coherent hop as across tiles, and a workgroup held by one tile
+// delays its next.)
 //
 // Publication: before a sweep every DQ slot it writes holds a NaN payload
 // (the sentinel) that arithmetic never produces.  A cell writes its 5 final
@@ -96,13 +100,12 @@ template <typename R>
 struct SweepArgs {
     int nC, nchunks;
     int ntiles;
-    const int* tile_chunk;                  // ntiles+1: first chunk of each tile
-    const int* run;                         // chunks in forward level order (flat schedule)
+    const int* tile_chunk;                  // ntiles+1: first chunk of each tile (device order)
+    const int* pos;                         // per chunk: its position in the flat order
     int flat;                               // 1: flat schedule, 0: tiles
-    const int2* chunk;                      // device order: first device cell, cell count
-    const int* cfs;                         // LPC face slots per device cell (-1 = none)
-    const int2* fcell;
-    const typename Real<R>::vec4* fgeo;     // unit normal + area
+    const int4* crun;                       // per position: first device cell, cell count, rep fwd, rep bwd
+    const int* snb;                         // LPC per device cell: neighbour across the slot's face
+    const typename Real<R>::vec4* sgeo;     // LPC per device cell: normal away from the cell + area
     const R* q;                             // operator state (cell-major, 5 per cell)
     const double* q64;                      // FP64 state the backward sweep updates
     const double* res;                      // forward: initial DQ (FP64 residual)
@@ -110,10 +113,12 @@ struct SweepArgs {
     R* dqf;                                 // forward DQ, 5 per cell
     R* dqb;                                 // backward (final) DQ, 5 per cell incl. ghosts (0)
     double* qout;                           // backward: updated state
-    const int* rep;                         // per chunk: dependency expected to be published last, -1 if none
     const int* depmask;                     // per cell: forward / backward dependency face slots
     int cell0;                              // cell with mesh id 0 (forward: divide only)
     int* err;
+    int* tickets;                           // this sweep's ticket counters (flat schedule)
+    int* tickets_next;                      // the other direction's, zeroed here for its next sweep
+    int ngroups;                            // ticket groups in use (<= kTicketGroups and <= workgroups)
 };
 
 // Adds the LPC per-face products of this lane's cell in slot order (the
@@ -150,42 +155,46 @@ __device__ __forceinline__ void load5r(const R* __restrict__ p, R v[5]) {
     for (int j = 0; j < 5; ++j) v[j] = p[j];
 }
 
-// One chunk t (device order) of a sweep.  Returns false when a wait
+// The part of the face term that does not depend on the neighbour's DQ:
+// the reference's FluxLUSGS3D split in two (FP64), or its increment form
+// (FP32 operator of mixed precision).
+template <typename R>
+using FacePre = std::conditional_t<sizeof(R) == sizeof(double), FluxPre<R>, FluxDeltaPre<R>>;
+
+// The chunk at position u of a sweep's order (the forward sweep walks the
+// positions up, the backward sweep down).  Returns false when a wait
 // expired (the error word is set).
+//
+// Latency: a chunk's operands come in three rounds of loads: its record;
+// then the latest dependency's first value (lane 0, requested first), the
+// cell's mask, slot neighbours and geometry, state, Diag and right-hand
+// side; then the neighbours' states together with their DQ when the latest
+// dependency was already published.  The DQ-independent half of every face
+// term is evaluated before waiting, so only the DQ-dependent half, the
+// sum, the limiter and the stores follow the dependency's arrival.
 template <bool FWD, int LPC, typename R>
-__device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowConst& K, int t) {
+__device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowConst& K, int u) {
     using B = typename Real<R>::bits;
     constexpr B kSent = Real<R>::kSent;
     const int lane = (int)threadIdx.x & 63;
     const int k = lane / LPC;  // cell of the chunk
     const int s = lane % LPC;  // face slot
     R* const poll = FWD ? A.dqf : A.dqb;
-    const int2 ch = A.chunk[t];
-    const bool active = k < ch.y;
-    const int c = ch.x + (active ? k : 0);  // idle lanes read the first cell and store nothing
+    const int4 rec = A.crun[u];
+    const int rep = FWD ? rec.z : rec.w;
+    B rv = 0;
+    if (lane == 0 && rep >= 0) rv = ld_dev<R>(poll + (size_t)rep * 5);
+    const bool active = k < rec.y;
+    const int c = rec.x + (active ? k : 0);  // idle lanes read the first cell and store nothing
     const unsigned m = (unsigned)A.depmask[c];
     unsigned kind = FWD ? (m & 0xffu) : ((m >> 8) & 0xffu);  // slots that take part
     const unsigned pendm = FWD ? kind : ((m >> 16) & 0xffu);  // slots with a DQ to wait for
     if (FWD && c == A.cell0) kind = 0;  // forward_lusgs.cpp:803-804: the first cell only divides
     const bool mine = active && ((kind >> s) & 1u);
-    bool pend = active && ((pendm >> s) & 1u) && mine;
-    // operands that do not depend on the sweep, loaded before waiting
-    R fnx = R(0), fny = R(0), fnz = R(0), far_ = R(0);
-    R qnb[5] = {R(0), R(0), R(0), R(0), R(0)};
-    R dv[5] = {R(0), R(0), R(0), R(0), R(0)};
-    int nb = 0;
-    if (mine) {
-        const int f = A.cfs[(size_t)c * LPC + s];
-        const int2 cc = A.fcell[f];
-        const typename Real<R>::vec4 g = A.fgeo[f];
-        const bool flip = (cc.y == c);  // normal oriented from c to its neighbour
-        nb = flip ? cc.x : cc.y;
-        fnx = flip ? -g.x : g.x;
-        fny = flip ? -g.y : g.y;
-        fnz = flip ? -g.z : g.z;
-        far_ = g.w;
-        load5r(A.q + (size_t)nb * 5, qnb);
-    }
+    bool pend = mine && ((pendm >> s) & 1u);
+    const size_t sl = (size_t)c * LPC + s;
+    const int nb = A.snb[sl];
+    const typename Real<R>::vec4 g = A.sgeo[sl];
     R qc[5], D[5];
     load5r(A.q + (size_t)c * 5, qc);
     const R dg = A.diag[c];
@@ -195,11 +204,34 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     } else {
         load5r(A.dqf + (size_t)c * 5, D);
     }
+    bool rep_seen = (rep < 0) || (__shfl(rv, 0, 64) != kSent);
+    R qnb[5] = {R(0), R(0), R(0), R(0), R(0)};
+    R dv[5] = {R(0), R(0), R(0), R(0), R(0)};
+    if (mine) load5r(A.q + (size_t)nb * 5, qnb);
+    if (rep_seen && pend) {  // the dependencies are most likely out: read them now
+        B b[5];
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            b[j] = ld_dev<R>(poll + (size_t)nb * 5 + j);
+            ok = ok && (b[j] != kSent);
+        }
+        if (ok) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) dv[j] = __builtin_bit_cast(R, b[j]);
+            pend = false;
+        }
+    }
+    FacePre<R> pre;
+    if (mine) {
+        if constexpr (sizeof(R) == sizeof(double))
+            flux_lusgs_pre<R>(pre, qnb, g.x, g.y, g.z, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga);
+        else
+            flux_lusgs_delta_pre<R>(pre, qnb, g.x, g.y, g.z, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga);
+    }
 
-    // one lane waits for the chunk's latest dependency
     unsigned spins = 0;
-    const int rep = A.rep[t];
-    if (rep >= 0) {
+    if (!rep_seen) {  // one lane waits for the chunk's latest dependency
         while (true) {
             B v = 0;
             if (lane == 0) v = ld_dev<R>(poll + (size_t)rep * 5);
@@ -213,7 +245,7 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
         }
     }
     // every lane reads its face's dependency, retrying the rare unpublished one
-    while (true) {
+    while (__ballot(pend) != 0) {
         if (pend) {
             B b[5];
             bool ok = true;
@@ -240,10 +272,10 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     if (mine) {
         R fl[5];
         if constexpr (sizeof(R) == sizeof(double))  // the reference's expression, bit for bit
-            flux_lusgs<R>(qnb, dv, fnx, fny, fnz, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga, fl);
+            flux_lusgs_post<R>(pre, dv, g.x, g.y, g.z, fl);
         else  // FP32 operator: the same function in increment form (no cancellation floor)
-            flux_lusgs_delta<R>(qnb, dv, fnx, fny, fnz, (R)K.gam, (R)K.p_bar, (R)K.lhs_omga, fl);
-        const R tmp = FWD ? R(0.5) * far_ : far_;
+            flux_lusgs_delta_post<R>(pre, dv, g.x, g.y, g.z, fl);
+        const R tmp = FWD ? R(0.5) * g.w : g.w;
 #pragma unroll
         for (int j = 0; j < 5; ++j) p[j] = tmp * fl[j];
     }
@@ -299,19 +331,30 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(SweepArgs<R> A, FlowConst
     const int W = (int)blockDim.x / 64;
     const int w = (int)threadIdx.x / 64;
     if (A.flat) {
-        const int G = (int)gridDim.x * W;
-        for (int u = (int)blockIdx.x * W + w; u < A.nchunks; u += G) {
-            const int t = A.run[FWD ? u : A.nchunks - 1 - u];
-            if (!sweep_chunk<FWD, LPC, R>(A, K, t)) return;
+        // the other direction's counters are idle during this sweep
+        if (blockIdx.x == 0 && threadIdx.x < kTicketGroups) A.tickets_next[threadIdx.x * kTicketStride] = 0;
+        const int ng = A.ngroups;
+        const int grp = (int)blockIdx.x % ng;
+        int* const ctr = A.tickets + grp * kTicketStride;
+        const int lane = (int)threadIdx.x & 63;
+        int tk = 0;
+        if (lane == 0) tk = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tk = __shfl(tk, 0, 64);
+        while (true) {
+            const int u = grp + ng * tk;
+            if (u >= A.nchunks) return;
+            int next = 0;  // taken now, used after this chunk: its latency overlaps the chunk
+            if (lane == 0) next = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!sweep_chunk<FWD, LPC, R>(A, K, FWD ? u : A.nchunks - 1 - u)) return;
+            tk = __shfl(next, 0, 64);
         }
-        return;
     }
     for (int kk = blockIdx.x; kk < A.ntiles; kk += gridDim.x) {
         const int k = FWD ? kk : A.ntiles - 1 - kk;
         const int c0 = A.tile_chunk[k], c1 = A.tile_chunk[k + 1];
         for (int j = w; j < c1 - c0; j += W) {
             const int t = FWD ? c0 + j : c1 - 1 - j;
-            if (!sweep_chunk<FWD, LPC, R>(A, K, t)) return;
+            if (!sweep_chunk<FWD, LPC, R>(A, K, A.pos[t])) return;
         }
     }
 }
