@@ -292,6 +292,45 @@ def mode_converge(args) -> dict:
             "lift_rel_error": abs(runs["mixed"]["lift"] - runs["fp64"]["lift"]) / abs(runs["fp64"]["lift"])}
 
 
+def mode_batch_converge(args) -> dict:
+    """The same seeded solves, all present at the start, each run to a
+    residual drop of --batch-drop in one device batch (converged solves
+    leave it), against the same solves run one at a time on the plain FP64
+    path."""
+    import numpy as np
+
+    import minifs_amd as mfs
+    from minifs_amd.serve import Request, Server, run_sequential
+
+    def reqs():
+        r = np.random.default_rng(args.batch_seed)
+        out = []
+        for i in range(args.solves):
+            nx, ny, nz = (int(r.integers(args.batch_lo, args.batch_hi + 1)) for _ in range(3))
+            bh, mach, alpha = float(r.uniform(0.02, 0.08)), float(r.uniform(0.3, 0.85)), float(r.uniform(-3.0, 5.0))
+            out.append(Request(i, 0.0, nx, ny, nz, mach, alpha, bh, int(r.integers(1, 2**31))))
+        return out
+
+    prec = mfs.MFS_MIXED if args.precision == "mixed" else mfs.MFS_FP64
+    batch_reqs = reqs()
+    srv = Server(prec, hbm_budget=args.hbm_budget, drop=args.batch_drop, max_iters=args.max_iters,
+                 check_every=args.check_every, workers=min(16, os.cpu_count() or 4))
+    served = srv.run(batch_reqs, progress=lambda r: print(f"[batch] solve {r.rid} {r.n_cells} cells {r.iters} it "
+                                                          f"{r.t_done:.2f} s", file=sys.stderr, flush=True))
+    print(json.dumps({"batched": served}), file=sys.stderr, flush=True)
+    out = {"metric": "batched solves to a residual drop", "mode": "batch", "batched": served,
+           "config": {"solves": args.solves, "hex_per_axis": [args.batch_lo, args.batch_hi], "seed": args.batch_seed,
+                      "precision": args.precision, "drop": args.batch_drop, "check_every": args.check_every}}
+    if not args.no_sequential:
+        seq_reqs = reqs()
+        seq = run_sequential(seq_reqs, drop=args.batch_drop, max_iters=args.max_iters, check_every=args.check_every,
+                             progress=lambda r: print(f"[sequential] solve {r.rid} {r.n_cells} cells {r.iters} it "
+                                                      f"{r.t_done:.2f} s", file=sys.stderr, flush=True))
+        out["sequential_fp64"] = seq
+        out["wall_speedup"] = round(seq["wall_s"] / served["wall_s"], 3)
+    return out
+
+
 def mode_batch(args) -> dict:
     """Independent solves with nx,ny,nz ~ U[lo,hi], bump ~ U[.02,.08], Mach ~
     U[.3,.85], AoA ~ U[-3,5] (seeded), stepped together."""
@@ -355,10 +394,13 @@ def mode_serve(args) -> dict:
                  check_every=args.check_every, workers=min(16, os.cpu_count() or 4))
     served = srv.run(reqs, progress=lambda r: print(f"[serve] request {r.rid} {r.n_cells} cells {r.iters} it "
                                                     f"{r.latency:.2f} s", file=sys.stderr, flush=True))
+    print(json.dumps({"served": served}), file=sys.stderr, flush=True)
     base = None
     if not args.no_sequential:
         again = make_requests(args.requests, args.rate, args.cells_lo, args.cells_hi, args.serve_seed)
-        base = run_sequential(again, drop=args.serve_drop, max_iters=args.max_iters, check_every=args.check_every)
+        base = run_sequential(again, drop=args.serve_drop, max_iters=args.max_iters, check_every=args.check_every,
+                              progress=lambda r: print(f"[sequential] request {r.rid} {r.n_cells} cells {r.iters} it "
+                                                       f"{r.latency:.2f} s", file=sys.stderr, flush=True))
         same = all(a.rho_sum == b.rho_sum and a.iters == b.iters
                    for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(again, key=lambda r: r.rid))) \
             if prec == mfs.MFS_FP64 else None
@@ -395,6 +437,8 @@ def main() -> int:
     ap.add_argument("--batch-lo", type=int, default=15)
     ap.add_argument("--batch-hi", type=int, default=60)
     ap.add_argument("--batch-seed", type=int, default=4001)
+    ap.add_argument("--batch-drop", type=float, default=1e-8,
+                    help="batch mode: run each solve to this residual drop (0: a fixed number of steps)")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--cells-lo", type=float, default=20000)
@@ -422,7 +466,7 @@ def main() -> int:
     if args.mode == "converge":
         line = mode_converge(args)
     elif args.mode == "batch":
-        line = mode_batch(args)
+        line = mode_batch_converge(args) if args.batch_drop > 0 else mode_batch(args)
     elif args.mode == "serve":
         line = mode_serve(args)
     else:
