@@ -79,6 +79,20 @@ def _cpu_mesh(td: str, grid_n: int, seed: int):
     return mesh, bc, nC
 
 
+def host_cpu() -> dict:
+    """The host's CPU model, logical CPU count and the OpenMP binding in use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+
+
 def cpu_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: float, seed: int) -> dict | None:
     """Times the reference's own serial build (oracle/_ref/ref_driver, built
     from the reference sources with g++ -O3) on a bounded sample of the same
@@ -104,6 +118,7 @@ def cpu_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: float, s
             "sample": f"reference serial build (oracle/_ref/ref_driver, g++ -O3 -ffp-contract=off), {iters} "
                       f"iterations of the {grid_n}^3x6 = {nC} cell box+bump mesh, same flow condition, "
                       f"loop time only ({t_loop:.1f} s)",
+            "host": host_cpu(),
         }
 
 
@@ -116,6 +131,7 @@ def cpu_port_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: flo
     if not os.path.exists(LIB):
         return None
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read by the OpenMP runtime when the library starts it
     with tempfile.TemporaryDirectory() as td:
         mesh, bc, nC = _cpu_mesh(td, grid_n, seed)
         o = Oracle(mesh, bc, mach, alpha, cfl)
@@ -128,6 +144,7 @@ def cpu_port_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: flo
         "kind": "port",
         "sample": f"C restatement, {threads} OpenMP threads, {iters} iterations of the {grid_n}^3x6 = {nC} cell "
                   f"box+bump mesh, loop time only ({t_loop:.1f} s)",
+        "host": host_cpu(),
     }
 
 

@@ -8,9 +8,10 @@
 // solve owns a whole number of XCD groups of 8 workgroups).  The per-cell arithmetic is the
 // single-solve kernels' own (the same device functions), so a solve gives
 // the same bits alone and in a batch.  The sweeps walk a list of (solve, u)
-// items, the u-th chunk in level order of every solve for u = 0, 1, ...;
-// dependencies never cross solves and each solve's chunks keep their order,
-// so the deadlock argument of the single-solve sweep holds.
+// items, the u-th chunk in level order of every solve for u = 0, 1, ...,
+// handed out by ticket groups; dependencies never cross solves and each
+// solve's chunks keep their order, so the deadlock argument of the
+// single-solve sweep holds.
 
 template <typename R>
 struct BatchSolve {
@@ -73,16 +74,29 @@ __global__ __launch_bounds__(kBlock) void k_b_clamp(const BatchSolve<R>* __restr
 
 template <bool FWD, int LPC, typename R>
 __global__ __launch_bounds__(kSweepBlock) void k_b_sweep(const BatchSolve<R>* __restrict__ T,
-                                                        const int2* __restrict__ items, int nitems) {
-    // items: (solve, u) with u the position in that solve's level order;
-    // each wave walks every G-th item, in order
-    const int W = (int)blockDim.x / 64;
-    const int G = (int)gridDim.x * W;
-    for (int it = (int)blockIdx.x * W + (int)threadIdx.x / 64; it < nitems; it += G) {
+                                                        const int2* __restrict__ items, int nitems, int* tickets,
+                                                        int* tickets_next, int ngroups) {
+    // items: (solve, u) with u the position in that solve's level order.
+    // Item i belongs to ticket group i mod ngroups; the waves of the
+    // workgroups of group g take its items in order from the group's
+    // counter, as in the single-solve sweep.
+    if (blockIdx.x == 0 && threadIdx.x < kTicketGroups) tickets_next[threadIdx.x * kTicketStride] = 0;
+    const int grp = (int)blockIdx.x % ngroups;
+    int* const ctr = tickets + grp * kTicketStride;
+    const int lane = (int)threadIdx.x & 63;
+    int tk = 0;
+    if (lane == 0) tk = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tk = __shfl(tk, 0, 64);
+    while (true) {
+        const int it = grp + ngroups * tk;
+        if (it >= nitems) return;
+        int next = 0;
+        if (lane == 0) next = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int2 su = items[it];
         const SweepArgs<R>& A = T[su.x].sw;
-        if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) continue;
-        if (!sweep_chunk<FWD, LPC, R>(A, T[su.x].K, FWD ? su.y : A.nchunks - 1 - su.y)) continue;
+        if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            (void)sweep_chunk<FWD, LPC, R>(A, T[su.x].K, FWD ? su.y : A.nchunks - 1 - su.y);
+        tk = __shfl(next, 0, 64);
     }
 }
 

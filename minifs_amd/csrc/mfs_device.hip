@@ -1005,6 +1005,7 @@ struct mfs_batch {
     hipEvent_t table_copied = nullptr;
     bool table_pending = false;
     int sweep_grid = 0;
+    int* tickets = nullptr;       // ticket counters of the batch sweeps: forward, then backward
 };
 
 namespace {
@@ -1224,13 +1225,16 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         if (!batch_ok(b, hipGetLastError(), "batch flux/time step")) return MFS_ERR_HIP;
         {
             int n_it = b->nitems;
-            void* args[] = {(void*)&Tf, (void*)&b->items_f, (void*)&n_it};
+            int ng = std::min(kTicketGroups, (int)sweep_grid.x);
+            int* tk_f = b->tickets;
+            int* tk_b = b->tickets + kTicketGroups * kTicketStride;
+            void* args[] = {(void*)&Tf, (void*)&b->items_f, (void*)&n_it, (void*)&tk_f, (void*)&tk_b, (void*)&ng};
             if (!batch_ok(b,
                           hipLaunchKernel((const void*)dev::k_b_sweep<true, LPC, R>, sweep_grid,
                                                      dim3(64 * kBatchSweepWaves), args, 0, b->stream),
                           "batch forward sweep"))
                 return MFS_ERR_HIP;
-            void* argsb[] = {(void*)&Tb, (void*)&b->items_b, (void*)&n_it};
+            void* argsb[] = {(void*)&Tb, (void*)&b->items_b, (void*)&n_it, (void*)&tk_b, (void*)&tk_f, (void*)&ng};
             if (!batch_ok(b,
                           hipLaunchKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
                                                      dim3(64 * kBatchSweepWaves), argsb, 0, b->stream),
@@ -1287,8 +1291,13 @@ extern "C" int mfs_batch_create(int precision, mfs_batch** out) {
     mfs_batch* b = new mfs_batch();
     b->precision = precision;
     (void)hipGetDevice(&b->device);
+    const size_t tbytes = sizeof(int) * 2 * kTicketGroups * kTicketStride;
     if (!batch_ok(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") ||
-        !batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate")) {
+        !batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate") ||
+        !batch_ok(b, hipMalloc(&b->tickets, tbytes), "hipMalloc(batch tickets)") ||
+        !batch_ok(b, hipMemsetAsync(b->tickets, 0, tbytes, b->stream), "memset batch tickets")) {
+        if (b->tickets) (void)hipFree(b->tickets);
+        if (b->table_copied) (void)hipEventDestroy(b->table_copied);
         if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return MFS_ERR_HIP;
@@ -1417,6 +1426,7 @@ extern "C" int mfs_batch_destroy(mfs_batch* b) {
     (void)hipFree(b->node_solve);
     (void)hipFree(b->items_f);
     (void)hipFree(b->items_b);
+    (void)hipFree(b->tickets);
     (void)hipHostFree(b->table_host);
     (void)hipFree(b->table_dev);
     (void)hipEventDestroy(b->table_copied);
