@@ -159,6 +159,15 @@ int mfs_inviscid_flux(mfs_ctx* ctx);      /* cal_flux.cpp:84-192, 1219-1254 (res
 int mfs_compute_timestep(mfs_ctx* ctx);   /* compute_timestep.cpp:66-122, 756-998, 1315-1338 */
 int mfs_forward_lusgs(mfs_ctx* ctx);      /* forward_lusgs.cpp:67-163 incl. sweeps and update */
 int mfs_pass_interface_data(mfs_ctx* ctx); /* load_grid.cpp:912-943: single domain, no-op */
+/* SetGhostVariables (load_grid.cpp:953-1212, inviscid branch) on the device
+ * state: the ghost cell of every boundary face from the cell in front of
+ * it.  The reference calls it once during set-up and leaves the
+ * per-iteration call commented out (main.cpp:283); mfs_ctx_set_ghost_refresh
+ * makes mfs_step call it after each update, before the gradient (off by
+ * default).  The FAR_FIELD subsonic branch uses pow, whose GPU result may
+ * differ from the host's in the last bit. */
+int mfs_set_ghost_variables(mfs_ctx* ctx);
+int mfs_ctx_set_ghost_refresh(mfs_ctx* ctx, int enable);
 int mfs_calculate_gradient(mfs_ctx* ctx); /* cal_gradient.cpp:87-167 */
 /* iter_done++ of main.cpp:289 and one history record (norms of the last
  * residual, running dt_max/dt_min). */
@@ -213,7 +222,7 @@ int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
  * group is bracketed with HIP events on the context stream. */
 int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
 /* names: "flux","timestep","flux_ts" (both fused, mfs_step),"reduce","diag","clamp",
- * "fwd_sweep","bwd_sweep","node","gradient";
+ * "fwd_sweep","bwd_sweep","ghosts","node","gradient";
  * returns accumulated milliseconds and launch count. */
 int mfs_ctx_get_timing(mfs_ctx* ctx, const char* name, double* ms, int* launches);
 int mfs_ctx_reset_timing(mfs_ctx* ctx);

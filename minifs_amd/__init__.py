@@ -154,6 +154,15 @@ class Solver:
     def pass_interface_data(self):
         check(lib().mfs_pass_interface_data(self._h), "mfs_pass_interface_data")
 
+    def set_ghost_variables(self):
+        """SetGhostVariables on the device state (load_grid.cpp:953-1212)."""
+        check(lib().mfs_set_ghost_variables(self._h), "mfs_set_ghost_variables")
+
+    def ghost_refresh(self, on: bool = True):
+        """Make step() call SetGhostVariables after each update (the call
+        main.cpp:283 leaves commented out)."""
+        check(lib().mfs_ctx_set_ghost_refresh(self._h, 1 if on else 0), "mfs_ctx_set_ghost_refresh")
+
     def calculate_gradient(self):
         check(lib().mfs_calculate_gradient(self._h), "mfs_calculate_gradient")
 

@@ -92,6 +92,8 @@ _SIGS = {
     "mfs_compute_timestep": (C.c_int, [C.c_void_p]),
     "mfs_forward_lusgs": (C.c_int, [C.c_void_p]),
     "mfs_pass_interface_data": (C.c_int, [C.c_void_p]),
+    "mfs_set_ghost_variables": (C.c_int, [C.c_void_p]),
+    "mfs_ctx_set_ghost_refresh": (C.c_int, [C.c_void_p, C.c_int]),
     "mfs_calculate_gradient": (C.c_int, [C.c_void_p]),
     "mfs_iteration_done": (C.c_int, [C.c_void_p]),
     "mfs_step": (C.c_int, [C.c_void_p, C.c_int]),

@@ -364,6 +364,19 @@ bool launch_gradient_t(mfs_ctx* ctx) {
     }
     return true;
 }
+// SetGhostVariables on the current state (and its FP32 snapshot in mixed
+// precision).
+bool launch_ghosts(mfs_ctx* ctx) {
+    if (ctx->nB == 0) return true;
+    Timed t(ctx, "ghosts");
+    const mfs_params& P = ctx->P;
+    const dev::GhostConst G{P.rho_s, P.u_s, P.v_s, P.w_s, P.p_s};
+    hipLaunchKernelGGL(dev::k_ghost, dim3(nblk(ctx->nB)), dim3(kBlock), 0, ctx->stream, ctx->nB, ctx->nC, ctx->gface,
+                       ctx->fcell, ctx->fgeo, ctx->btype, ctx->K, G, ctx->q[ctx->cur],
+                       ctx->precision == MFS_MIXED ? ctx->q32 : nullptr);
+    return check_launch(ctx, "k_ghost");
+}
+
 bool launch_gradient(mfs_ctx* ctx) {
     if (ctx->precision == MFS_MIXED)
         return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx) : launch_gradient_t<6, float>(ctx);
@@ -551,7 +564,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     }
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err") &&
-         dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets");
+         dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets") &&
+         dalloc(ctx, ctx->gface, (size_t)L.nB, "ghost faces");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
         // resident workgroups of the sweep launch
@@ -570,9 +584,11 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     if (ok) {
         std::vector<int2> fc(L.nF);
         std::vector<double2> fd(L.nF);
+        std::vector<int> gf(std::max(L.nB, 1), 0);
         for (int i = 0; i < L.nF; ++i) {
             fc[i] = make_int2(L.c1[i], L.c2[i]);
             fd[i] = make_double2(L.dn1[i], L.dn2[i]);
+            if (L.c2[i] >= L.nC) gf[L.c2[i] - L.nC] = i;
         }
         std::vector<double> qa(5 * n);
         for (size_t c = 0; c < n; ++c)
@@ -596,7 +612,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              upload(ctx, ctx->tile_chunk, L.tile_chunk.data(), L.tile_chunk.size(), "upload tile_chunk") &&
              upload(ctx, ctx->snb, L.snb.data(), L.snb.size(), "upload snb") &&
              upload(ctx, ctx->sgeo, reinterpret_cast<const double4*>(L.sgeo.data()), L.snb.size(), "upload sgeo") &&
-             upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask");
+             upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask") &&
+             upload(ctx, ctx->gface, gf.data(), (size_t)L.nB, "upload gface");
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
         std::vector<float> f32;
@@ -706,6 +723,20 @@ extern "C" int mfs_forward_lusgs(mfs_ctx* ctx) {
 
 extern "C" int mfs_pass_interface_data(mfs_ctx* ctx) { return usable(ctx) ? MFS_OK : MFS_ERR_STATE; }
 
+extern "C" int mfs_set_ghost_variables(mfs_ctx* ctx) {
+    if (!usable(ctx)) return MFS_ERR_STATE;
+    return launch_ghosts(ctx) ? MFS_OK : MFS_ERR_HIP;
+}
+
+extern "C" int mfs_ctx_set_ghost_refresh(mfs_ctx* ctx, int enable) {
+    if (!ctx) {
+        set_error("mfs_ctx_set_ghost_refresh: null context");
+        return MFS_ERR_ARG;
+    }
+    ctx->ghost_refresh = enable ? 1 : 0;
+    return MFS_OK;
+}
+
 extern "C" int mfs_calculate_gradient(mfs_ctx* ctx) {
     if (!usable(ctx)) return MFS_ERR_STATE;
     return launch_gradient(ctx) ? MFS_OK : MFS_ERR_HIP;
@@ -738,6 +769,8 @@ extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
         if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
         // ForwardLUSGS: sweeps + update
         if (!launch_sweeps(ctx)) return MFS_ERR_HIP;
+        // SetGhostVariables, when enabled (main.cpp:283 has the call commented out)
+        if (ctx->ghost_refresh && !launch_ghosts(ctx)) return MFS_ERR_HIP;
         // CalculateGradient
         if (!launch_gradient(ctx)) return MFS_ERR_HIP;
         ctx->hist_n++;
@@ -1240,6 +1273,17 @@ int batch_step_t(mfs_batch* b, int nsteps) {
                                                      dim3(64 * kBatchSweepWaves), argsb, 0, b->stream),
                           "batch backward sweep"))
                 return MFS_ERR_HIP;
+        }
+        for (int a = 0; a < ns; ++a) {  // SetGhostVariables for the solves that asked for it
+            mfs_ctx* c = b->slots[b->active[a]];
+            if (c->ghost_refresh && c->nB > 0) {
+                const int cur = c->cur ^ (i & 1);
+                const mfs_params& P = c->P;
+                const dev::GhostConst G{P.rho_s, P.u_s, P.v_s, P.w_s, P.p_s};
+                hipLaunchKernelGGL(dev::k_ghost, dim3(nblk(c->nB)), dim3(kBlock), 0, b->stream, c->nB, c->nC,
+                                   c->gface, c->fcell, c->fgeo, c->btype, c->K, G, c->q[1 - cur],
+                                   MIXED ? c->q32 : nullptr);
+            }
         }
         hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, b->stream, Tf, b->node_solve);
         for (int a = 0; a < ns; ++a) {  // symmetry-plane projection, only where a solve has SYMM nodes

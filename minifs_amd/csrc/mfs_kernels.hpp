@@ -101,6 +101,111 @@ __global__ void k_fill_f(float* p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
+// SetGhostVariables (src/Kernels/load_grid.cpp:953-1212), inviscid branch:
+// the ghost state behind each boundary face from the state of the cell in
+// front of it.  One thread per boundary face k; the ghost is cell nC+k and
+// gface[k] the device id of its face.  WALL and SYMM mirror the velocity
+// in the face plane; FAR_FIELD takes the Riemann-invariant state
+// (supersonic in/outflow, subsonic in/outflow with the entropy of the
+// upwind side) and extrapolates it to the ghost.  Expressions and their
+// order follow the reference; the FAR_FIELD subsonic branch calls pow,
+// whose GPU result may differ from the host's in the last bit.
+struct GhostConst {
+    double rho_s, u_s, v_s, w_s, p_s;  // freestream (p gauge)
+};
+
+__global__ __launch_bounds__(kBlock) void k_ghost(int nB, int nC, const int* __restrict__ gface,
+                                                  const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
+                                                  const int* __restrict__ btype, FlowConst K, GhostConst G,
+                                                  double* __restrict__ q, float* __restrict__ q32) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= nB) return;
+    const int type = btype[k];
+    if (type == BC_INTERFACE) return;
+    const int f = gface[k];
+    const int c1 = fcell[f].x;
+    const size_t o1 = (size_t)c1 * 5, o2 = (size_t)(nC + k) * 5;
+    const double4 g = fgeo[f];
+    const double xfn = g.x, yfn = g.y, zfn = g.z;
+    const double gam = K.gam, p_bar = K.p_bar;
+    const double r1 = q[o1], u1 = q[o1 + 1], v1 = q[o1 + 2], w1 = q[o1 + 3], p1 = q[o1 + 4];
+    double rho2 = q[o2], u2 = q[o2 + 1], v2 = q[o2 + 2], w2 = q[o2 + 3], p2 = q[o2 + 4];
+    if (type == BC_WALL || type == BC_SYMM) {
+        p2 = p1;
+        rho2 = r1;
+        const double vn = 2. * (xfn * u1 + yfn * v1 + zfn * w1);
+        u2 = u1 - vn * xfn;
+        v2 = v1 - vn * yfn;
+        w2 = w1 - vn * zfn;
+    } else if (type == BC_FAR_FIELD) {
+        const double um = G.u_s, vm = G.v_s, wm = G.w_s;
+        const double up = u1, vp = v1, wp = w1;
+        const double rhom = G.rho_s, pm = G.p_s + p_bar;
+        const double rhop = r1, pp = p1 + p_bar;
+        const double norm_of_uvw = sqrt(um * um + vm * vm + wm * wm);
+        const double vnm = xfn * um + yfn * vm + zfn * wm;
+        const double vnp = xfn * up + yfn * vp + zfn * wp;
+        const double cm = sqrt(gam * pm / rhom);
+        const double cp = sqrt(gam * pp / rhop);
+        const double riemm = vnm - 2. * cm / (gam - 1.);
+        const double riemp = vnp + 2. * cp / (gam - 1.);
+        const double vnb = 0.5 * (riemp + riemm);
+        const double cb = 0.25 * (riemp - riemm) * (gam - 1.);
+        if (fabs(vnb / cb) > 1.) {
+            if (vnb <= 0.0) {
+                rho2 = rhom;
+                u2 = um;
+                v2 = vm;
+                w2 = wm;
+                p2 = pm;
+            } else {
+                rho2 = rhop;
+                u2 = up;
+                v2 = vp;
+                w2 = wp;
+                p2 = pp;
+            }
+        } else {
+            const double rela_vnb = vnb / norm_of_uvw;
+            if (rela_vnb <= -0.0 || rela_vnb > 0.0) {
+                const bool in = rela_vnb <= -0.0;
+                const double bu = in ? um : up, bv = in ? vm : vp, bw = in ? wm : wp, bvn = in ? vnm : vnp;
+                const double bp = in ? pm : pp, brho = in ? rhom : rhop;
+                const double vtx = bu - bvn * xfn, vty = bv - bvn * yfn, vtz = bw - bvn * zfn;
+                const double entr = bp / pow(brho, gam);
+                rho2 = pow((cb * cb / (entr * gam)), double(1. / (gam - 1.)));
+                u2 = vtx + vnb * xfn;
+                v2 = vty + vnb * yfn;
+                w2 = vtz + vnb * zfn;
+                p2 = cb * cb * rho2 / gam;
+            } else {
+                rho2 = 0.5 * (rhop + rhom);
+                u2 = 0.5 * (up + um);
+                v2 = 0.5 * (vp + vm);
+                w2 = 0.5 * (wp + wm);
+                p2 = 0.5 * (pp + pm);
+            }
+        }
+        rho2 = 2 * rho2 - rhop;
+        u2 = 2 * u2 - up;
+        v2 = 2 * v2 - vp;
+        w2 = 2 * w2 - wp;
+        p2 = 2 * p2 - pp;
+        p2 -= p_bar;
+    }  // other types: the reference reports an error, keeps the ghost state and clamps it below
+    rho2 = smax(rho2, K.rho_min);
+    rho2 = smin(rho2, K.rho_max);
+    p2 = smax(p2, K.p_min);
+    p2 = smin(p2, K.p_max);
+    const double out[5] = {rho2, u2, v2, w2, p2};
+#pragma unroll
+    for (int j = 0; j < 5; ++j) q[o2 + j] = out[j];
+    if (q32) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) q32[o2 + j] = (float)out[j];
+    }
+}
+
 // Cell-face slots of one cell: device face ids, padded with the first face.
 template <int MAXF>
 __device__ __forceinline__ int cell_faces(const int* __restrict__ cfo, const int* __restrict__ cf, int c,

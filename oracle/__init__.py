@@ -30,6 +30,9 @@ def lib():
         L.or_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_get.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]
         L.or_sizes.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 4
+        L.or_set_ghost_refresh.argtypes = [C.c_void_p, C.c_int]
+        L.or_set.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]
+        L.or_set_ghosts.argtypes = [C.c_void_p]
         L.or_free.argtypes = [C.c_void_p]
         L.or_free.restype = None
         L.or_error.restype = C.c_char_p
@@ -69,6 +72,16 @@ class Oracle:
         out = np.empty(sizes[name])
         self._check(lib().or_get(self._h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
         return out
+
+    def set_ghost_refresh(self, on: bool):
+        self._check(lib().or_set_ghost_refresh(self._h, 1 if on else 0))
+
+    def set(self, name: str, values: np.ndarray):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(lib().or_set(self._h, name.encode(), a.ctypes.data_as(C.POINTER(C.c_double)), a.size))
+
+    def set_ghosts(self):
+        self._check(lib().or_set_ghosts(self._h))
 
     def close(self):
         if self._h:

@@ -54,6 +54,7 @@ struct or_grid {
     int *flev_off, *flev_cells, nflev;
     int iter_done;
     double dt_max, dt_min, cfl_min;
+    int ghost_refresh; /* SetGhostVariables after each update (the commented main.cpp:283 call) */
     Par P;
 };
 
@@ -868,6 +869,9 @@ static void run_iteration(or_grid* g, int threads) {
         p[i] = dmax(p[i], P->p_min);
         p[i] = dmin(p[i], P->p_max);
     }
+    /* PassInterfaceData: single domain, nothing to pass.  SetGhostVariables
+     * when enabled, where main.cpp:283 has it commented out. */
+    if (g->ghost_refresh) ghosts(g);
     /* CalculateGradient (cal_gradient.cpp:87-167): CompNodeVar3D_dist + CompGradientQ_Gauss_Node, per variable */
     double* qn = malloc(8 * (size_t)nN);
     memset(g->dqdx, 0, 8 * 5 * n);
@@ -951,6 +955,28 @@ int or_run(or_grid* g, int iters, int threads, double* hist, double* t_loop) {
         }
     }
     if (t_loop) *t_loop = total;
+    return 0;
+}
+
+int or_set_ghost_refresh(or_grid* g, int on) {
+    g->ghost_refresh = on ? 1 : 0;
+    return 0;
+}
+
+int or_set(or_grid* g, const char* name, const double* src, size_t count) {
+    const char* st[5] = {"rho", "u", "v", "w", "p"};
+    const size_t n = (size_t)g->nC + g->nB;
+    for (int j = 0; j < 5; ++j)
+        if (strcmp(name, st[j]) == 0) {
+            if (count != n) return fail("or_set: size mismatch");
+            memcpy(g->q[j], src, sizeof(double) * n);
+            return 0;
+        }
+    return fail("or_set: only the state arrays rho u v w p can be set");
+}
+
+int or_set_ghosts(or_grid* g) {
+    ghosts(g);
     return 0;
 }
 

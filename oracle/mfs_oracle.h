@@ -38,6 +38,13 @@ int or_run(or_grid* g, int iters, int threads, double* hist, double* t_loop);
  * dqdy dqdz (5 (nC+nB)), xfn..zfn area xfc..zfc (nF), xcc..zcc vol (nC+nB) */
 int or_get(const or_grid* g, const char* name, double* dst, size_t count);
 int or_sizes(const or_grid* g, int* nC, int* nF, int* nB, int* nN);
+/* SetGhostVariables after each iteration's update (the call main.cpp:283
+ * leaves commented out); off after or_preprocess */
+int or_set_ghost_refresh(or_grid* g, int on);
+/* overwrites a state array (rho u v w p, nC+nB) */
+int or_set(or_grid* g, const char* name, const double* src, size_t count);
+/* SetGhostVariables (load_grid.cpp:953-1212) on the current state */
+int or_set_ghosts(or_grid* g);
 void or_free(or_grid* g);
 const char* or_error(void);
 

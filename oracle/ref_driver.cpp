@@ -108,9 +108,12 @@ static void dump_state(PolyGrid* g, const std::string& dir, int it) {
 
 int main(int argc, char** argv) {
     if (argc < 9) {
-        fprintf(stderr, "usage: %s MESH BC OUTDIR MACH ALPHA CFL ITERS DUMPLIST\n", argv[0]);
+        fprintf(stderr, "usage: %s MESH BC OUTDIR MACH ALPHA CFL ITERS DUMPLIST [ghosts]\n", argv[0]);
         return 2;
     }
+    // "ghosts": call SetGhostVariables after PassInterfaceData in every
+    // iteration, where src/main.cpp:283 has the call commented out
+    const bool ghost_refresh = argc >= 10 && std::string(argv[9]) == "ghosts";
     const std::string mesh = argv[1], bcf = argv[2], out = argv[3];
     const double m_in = atof(argv[4]), a_in = atof(argv[5]), cfl = atof(argv[6]);
     const int iters = atoi(argv[7]);
@@ -164,6 +167,7 @@ int main(int argc, char** argv) {
         ComputeTimeStep(grid);
         ForwardLUSGS(grid, 0);
         PassInterfaceData(grid);
+        if (ghost_refresh) SetGhostVariables(grid);
         CalculateGradient(grid);
         iter_done++;
         auto t1 = std::chrono::steady_clock::now();

@@ -150,3 +150,37 @@ def test_native_driver_matches_python_api(entry_points):
         np.testing.assert_allclose(got, h[-1], rtol=1e-12, atol=0)
     else:
         np.testing.assert_array_equal(got, h[-1])
+
+
+def test_set_ghost_variables_matches_oracle(tmp_path):
+    """mfs_set_ghost_variables on a state the iterations produced: interior
+    cells unchanged, ghosts as the oracle's SetGhostVariables sets them
+    (to 1e-14 of each field's scale: the pow of the subsonic far field may
+    differ in the last bit)."""
+    mach, alpha, cfl = 0.5, 2.0, 10.0
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
+    g = mfs.Grid.box_bump(6).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    g.preprocess(p)
+    nC = g.info["nTCell"]
+    s = mfs.Solver(g, p)
+    s.step(4)
+    before = {nm: s.field(nm) for nm in ("rho", "u", "v", "w", "p")}
+    s.set_ghost_variables()
+    after = {nm: s.field(nm) for nm in ("rho", "u", "v", "w", "p")}
+    o = Oracle(mesh, bc, mach, alpha, cfl)
+    for nm, v in before.items():
+        o.set(nm, v)
+    o.set_ghosts()
+    changed = False
+    for nm in before:
+        np.testing.assert_array_equal(after[nm][:nC], before[nm][:nC], err_msg=nm)
+        want = o.get(nm)
+        scale = np.max(np.abs(want)) if nm != "p" else p.p_bar
+        np.testing.assert_allclose(after[nm][nC:], want[nC:], rtol=0, atol=1e-14 * scale, err_msg=nm)
+        changed |= not np.array_equal(after[nm][nC:], before[nm][nC:])
+    assert changed  # the ghosts did follow the interior
+    o.close()
+    s.close()
+    g.close()

@@ -77,3 +77,29 @@ def test_batch_polyhedral_family():
     with pytest.raises(mfs.MfsError):
         b.add(g, p)
     b.close()
+
+
+@pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
+def test_batch_ghost_refresh_equals_standalone(precision):
+    """SetGhostVariables after each update (mfs_ctx_set_ghost_refresh) runs
+    for the batch members that ask for it, with the same bits as alone."""
+    print(f"[batch] ghost refresh precision={precision}", flush=True)
+    cases = [_case(6, 0.5, 2.0, 5), _case(7, 0.7, -1.0, 6)]
+    refs = []
+    for g, p in cases:
+        s = mfs.Solver(g, p, precision=precision)
+        s.ghost_refresh(True)
+        s.step(7)
+        refs.append(({nm: s.field(nm) for nm in FIELDS}, s.history()))
+        s.close()
+    b = mfs.Batch(precision)
+    members = [b.add(g, p) for g, p in cases]
+    for m in members:
+        m.ghost_refresh(True)
+    b.step(7)
+    b.sync()
+    for m, (ref, href) in zip(members, refs):
+        _same(m, ref, href)
+    b.close()
+    for g, _ in cases:
+        g.close()

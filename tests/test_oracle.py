@@ -102,6 +102,69 @@ def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver):
         np.testing.assert_array_equal(o.get(nm), load_state(str(tmp_path / "r"), 6, nm), err_msg=nm)
 
 
+def test_oracle_ghost_refresh_matches_reference_build(tmp_path, ref_driver):
+    """SetGhostVariables after every update (the call main.cpp:283 leaves
+    commented out): the oracle against the reference build running its own
+    SetGhostVariables each iteration.  Mach 0.5 keeps the far field
+    subsonic, so the pow branch runs on every far-field face."""
+    import minifs_amd as mfs
+    from tests.refrun import load_state, run_reference
+
+    g = mfs.Grid.box_bump(5, 4, 4, seed=11).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    ref = run_reference(ref_driver, mesh, bc, str(tmp_path / "r"), 0.5, 2.0, 10.0, 8, dumps=(8,), ghost_refresh=True)
+    plain = run_reference(ref_driver, mesh, bc, str(tmp_path / "p"), 0.5, 2.0, 10.0, 8, dumps=(8,))
+    # the refresh changes the run: the ghosts follow the interior
+    assert not np.array_equal(load_state(str(tmp_path / "r"), 8, "rho"), load_state(str(tmp_path / "p"), 8, "rho"))
+    for threads in (0, 3):
+        o = Oracle(mesh, bc, 0.5, 2.0, 10.0)
+        o.set_ghost_refresh(True)
+        h, _ = o.run(8, threads)
+        np.testing.assert_array_equal(h, ref["history"])
+        for nm in FIELDS:
+            np.testing.assert_array_equal(o.get(nm), load_state(str(tmp_path / "r"), 8, nm), err_msg=nm)
+        o.close()
+
+
+def test_ghost_states_wall_mirror_and_far_field_bounds():
+    """Closed forms of SetGhostVariables on a state the iterations produced:
+    a WALL ghost has the cell's density and pressure and the cell's velocity
+    mirrored in the face plane; every ghost lies inside the clamps."""
+    import minifs_amd as mfs
+
+    o = box4_oracle()
+    o.run(5, 0)
+    o.set_ghosts()
+    c = MAN["cases"]["box4"]
+    h = mfs.Grid.load(os.path.join(GOLD, "box4.mfl"), os.path.join(GOLD, "box4.bc"))
+    h.preprocess(mfs.make_params(mach=c["mach"], alpha=c["alpha"], cfl=c["cfl"]))  # ghost ids nC+k in f2c
+    f2c = h.array("f2c").reshape(-1, 2)
+    bt = h.array("bctype")
+    nB, nC = o.nB, o.nC
+    q = {nm: o.get(nm) for nm in ("rho", "u", "v", "w", "p")}
+    xn, yn, zn = o.get("xfn")[:nB], o.get("yfn")[:nB], o.get("zfn")[:nB]
+    c1, c2 = f2c[:nB, 0], f2c[:nB, 1]
+    assert np.array_equal(c2, nC + np.arange(nB))
+    wall = bt == 3
+    assert wall.any() and (bt == 6).any()
+    assert np.array_equal(q["rho"][c2[wall]], q["rho"][c1[wall]])
+    assert np.array_equal(q["p"][c2[wall]], q["p"][c1[wall]])
+    vn1 = xn * q["u"][c1] + yn * q["v"][c1] + zn * q["w"][c1]
+    vn2 = xn * q["u"][c2] + yn * q["v"][c2] + zn * q["w"][c2]
+    speed = np.sqrt(q["u"][c1] ** 2 + q["v"][c1] ** 2 + q["w"][c1] ** 2)
+    np.testing.assert_allclose(vn2[wall], -vn1[wall], rtol=0, atol=1e-13 * speed[wall].max())
+    for comp, nrm in (("u", xn), ("v", yn), ("w", zn)):  # tangential part unchanged
+        t1 = q[comp][c1] - vn1 * nrm
+        t2 = q[comp][c2] - vn2 * nrm
+        np.testing.assert_allclose(t2[wall], t1[wall], rtol=0, atol=1e-13 * speed[wall].max())
+    d = derived(MAN["cases"]["box4"]["mach"], MAN["cases"]["box4"]["alpha"])
+    assert np.all(q["rho"][nC:] >= d["rho_min"]) and np.all(q["rho"][nC:] <= d["rho_max"])
+    assert np.all(q["p"][nC:] >= d["p_min"]) and np.all(q["p"][nC:] <= d["p_max"])
+    o.close()
+    h.close()
+
+
 # ---------------------------------------------------------------- closed forms
 def derived(mach, alpha):
     d = np.zeros(14)
