@@ -40,6 +40,7 @@ struct BatchSolve {
     const int *fno, *fn, *btype;
     const typename Real<R>::vec4* fgeo_r;
     R *gx, *gy, *gz;
+    const int* snode;             // tetrahedral meshes: slot nodes (nullptr otherwise)
     // sweeps
     SweepArgs<R> sw;
 };
@@ -116,6 +117,10 @@ __global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restri
     const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
-    grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
-                       S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
+    if (MAXF == 4 && S.snode)
+        grad_tet_body<R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.sw.snb, S.sw.sgeo, S.snode, S.sw.depmask,
+                         S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
+    else
+        grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
+                           S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
 }

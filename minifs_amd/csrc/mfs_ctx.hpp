@@ -88,6 +88,7 @@ struct mfs_ctx {
     int* depmask = nullptr;
     int* tickets = nullptr;  // ticket counters of the flat sweep schedule: forward, then backward
     int* gface = nullptr;    // nB: device face of each ghost cell nC+k
+    int* snode = nullptr;    // tetrahedral meshes: 3 face nodes per slot, 12 per cell
     int ghost_refresh = 0;   // SetGhostVariables after each update in mfs_step
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;

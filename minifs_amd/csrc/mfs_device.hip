@@ -357,10 +357,21 @@ bool launch_gradient_t(mfs_ctx* ctx) {
     }
     {
         Timed t(ctx, "gradient");
-        hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
-                           ctx->cfo, ctx->cf, ctx->fcell, fgeo, ctx->fno, ctx->fn, ctx->btype, q, qn, ctx->vol, gx, gy,
-                           gz);
-        if (!check_launch(ctx, "k_grad")) return false;
+        if (MAXF == 4 && ctx->snode) {
+            const typename dev::Real<R>::vec4* sgeo;
+            if constexpr (F64)
+                sgeo = ctx->sgeo;
+            else
+                sgeo = ctx->sgeo32;
+            hipLaunchKernelGGL(dev::k_grad_tet<R>, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
+                               ctx->snb, sgeo, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz);
+            if (!check_launch(ctx, "k_grad_tet")) return false;
+        } else {
+            hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+                               n, ctx->cfo, ctx->cf, ctx->fcell, fgeo, ctx->fno, ctx->fn, ctx->btype, q, qn, ctx->vol,
+                               gx, gy, gz);
+            if (!check_launch(ctx, "k_grad")) return false;
+        }
     }
     return true;
 }
@@ -566,6 +577,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
          dalloc(ctx, ctx->err, 4, "err") &&
          dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets") &&
          dalloc(ctx, ctx->gface, (size_t)L.nB, "ghost faces");
+    if (!L.snode.empty()) ok = ok && dalloc(ctx, ctx->snode, L.snode.size(), "slot nodes");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
         // resident workgroups of the sweep launch
@@ -613,7 +625,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              upload(ctx, ctx->snb, L.snb.data(), L.snb.size(), "upload snb") &&
              upload(ctx, ctx->sgeo, reinterpret_cast<const double4*>(L.sgeo.data()), L.snb.size(), "upload sgeo") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask") &&
-             upload(ctx, ctx->gface, gf.data(), (size_t)L.nB, "upload gface");
+             upload(ctx, ctx->gface, gf.data(), (size_t)L.nB, "upload gface") &&
+             (L.snode.empty() || upload(ctx, ctx->snode, L.snode.data(), L.snode.size(), "upload snode"));
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
         std::vector<float> f32;
@@ -1141,6 +1154,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     E.q32 = c->q32;
     E.diag32 = c->diag32;
     E.K = c->K;
+    E.snode = c->snode;
     E.nlo = c->nlo;
     E.ell_off = c->ell_off;
     E.ell_ab = c->ell_ab;

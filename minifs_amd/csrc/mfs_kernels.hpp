@@ -734,6 +734,87 @@ __device__ __forceinline__ void grad_body(int c, int nC, int n, const int* __res
     }
 }
 
+// The same gradient for tetrahedral cells, from per-slot tables laid out
+// by cell: the neighbour across each face (snb), the face's unit normal
+// pointing away from the cell with its area (sgeo), the face's three nodes
+// in the face's own order (snode), and in bits 24-27 of depmask the
+// boundary slots that take the cell/ghost average.  A cell reads 4 + 16 +
+// 12 contiguous values and then gathers its node values, instead of
+// walking face list -> face -> nodes.  Subtracting q_f*A*n for a face whose
+// second cell this is equals adding q_f*A*(-n) exactly, so the sums are
+// the same bits as grad_body's.
+template <typename R>
+__device__ __forceinline__ void grad_tet_body(int c, int nC, int n, const int* __restrict__ snb,
+                                              const typename Real<R>::vec4* __restrict__ sgeo,
+                                              const int* __restrict__ snode, const int* __restrict__ depmask,
+                                              const double* __restrict__ q, const R* __restrict__ qn,
+                                              const double* __restrict__ vol, R* __restrict__ gx,
+                                              R* __restrict__ gy, R* __restrict__ gz) {
+    if (c >= nC) return;
+    const unsigned fca = (unsigned)depmask[c] >> 24;
+    const int4 nb4 = reinterpret_cast<const int4*>(snb)[c];
+    const int nbs[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    typename Real<R>::vec4 g[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) g[s] = sgeo[(size_t)c * 4 + s];
+    const int4* sn = reinterpret_cast<const int4*>(snode + (size_t)c * 12);
+    const int4 t0 = sn[0], t1 = sn[1], t2 = sn[2];
+    const int tn[12] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w, t2.x, t2.y, t2.z, t2.w};
+    R ax[5] = {R(0), R(0), R(0), R(0), R(0)}, ay[5] = {R(0), R(0), R(0), R(0), R(0)},
+      az[5] = {R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        R qs[5];
+        if ((fca >> s) & 1u) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                qs[j] = R(0.5) * ((R)q[(size_t)c * 5 + j] + (R)q[(size_t)nbs[s] * 5 + j]);
+        } else {
+            R sum[5] = {R(0), R(0), R(0), R(0), R(0)};
+            R v0[5], v1[5], v2[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                v0[j] = qn[(size_t)tn[3 * s] * 5 + j];
+                v1[j] = qn[(size_t)tn[3 * s + 1] * 5 + j];
+                v2[j] = qn[(size_t)tn[3 * s + 2] * 5 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                sum[j] += v0[j];
+                sum[j] += v1[j];
+                sum[j] += v2[j];
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) qs[j] = sum[j] / R(3.0);
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const R sa = qs[j] * g[s].w;
+            ax[j] += sa * g[s].x;
+            ay[j] += sa * g[s].y;
+            az[j] += sa * g[s].z;
+        }
+    }
+    const R vc = (R)vol[c];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        gx[(size_t)j * n + c] = ax[j] / vc;
+        gy[(size_t)j * n + c] = ay[j] / vc;
+        gz[(size_t)j * n + c] = az[j] / vc;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_grad_tet(int nC, int n, const int* __restrict__ snb,
+                                                     const typename Real<R>::vec4* __restrict__ sgeo,
+                                                     const int* __restrict__ snode, const int* __restrict__ depmask,
+                                                     const double* __restrict__ q, const R* __restrict__ qn,
+                                                     const double* __restrict__ vol, R* __restrict__ gx,
+                                                     R* __restrict__ gy, R* __restrict__ gz) {
+    grad_tet_body<R>(xcd_block() * kBlock + (int)threadIdx.x, nC, n, snb, sgeo, snode, depmask, q, qn, vol, gx, gy,
+                     gz);
+}
+
 template <int MAXF, typename R>
 __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __restrict__ cfo,
                                                  const int* __restrict__ cf, const int2* __restrict__ fcell,

@@ -142,18 +142,26 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
         int pos = 0;
         for (int d = 0; d < nC; ++d) {
             const int c = L.cell_of_dev[d];
-            int s = 0, mask = 0;
+            int s = 0;
+            unsigned mask = 0;
             for (int k = mcfo[c]; k < mcfo[c + 1]; ++k, ++s) {
                 const int f = mcf[k];
                 L.cf[pos++] = dface[f];
                 const int nb = other(c, f);
-                if (nb < c) mask |= 1 << s;  // ghosts have nb >= nC > c
+                if (nb < c) mask |= 1u << s;  // ghosts have nb >= nC > c
                 if (nb > c) {
-                    mask |= 1 << (8 + s);
-                    if (nb < nC) mask |= 1 << (16 + s);
+                    mask |= 1u << (8 + s);
+                    if (nb < nC) mask |= 1u << (16 + s);
+                }
+                // gradient: boundary faces other than SYMM and INTERFACE take
+                // the face-centre average of cell and ghost instead of the
+                // node mean (cal_gradient.cpp:1740-1795)
+                if (nb >= nC) {
+                    const int t = L.btype[nb - nC];
+                    if (t != BC_SYMM && t != BC_INTERFACE) mask |= 1u << (24 + s);
                 }
             }
-            L.depmask[d] = mask;
+            L.depmask[d] = (int)mask;
             L.cfo[d + 1] = pos;
         }
     }
@@ -447,6 +455,21 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
             L.crun[4 * (size_t)u + 1] = L.chunk[2 * t + 1];
             L.crun[4 * (size_t)u + 2] = L.rep_f[t];
             L.crun[4 * (size_t)u + 3] = L.rep_b[t];
+        }
+        // tetrahedral meshes: each slot's three face nodes in the face's own
+        // order (the gradient's node mean adds them in that order)
+        L.snode.clear();
+        if (L.max_faces == 4) {
+            bool tri = true;
+            L.snode.assign(12 * (size_t)nC, 0);
+            for (int d = 0; d < nC && tri; ++d)
+                for (int sl = 0; sl < 4 && tri; ++sl) {
+                    const int f = L.cfs[(size_t)d * 4 + sl];
+                    tri = f >= 0 && L.fno[f + 1] - L.fno[f] == 3;
+                    if (tri)
+                        for (int i = 0; i < 3; ++i) L.snode[12 * (size_t)d + 3 * sl + i] = L.fn[L.fno[f] + i];
+                }
+            if (!tri) L.snode.clear();  // not all triangles: the gradient keeps its face-list path
         }
         L.snb.resize((size_t)nC * L.lpc);
         L.sgeo.assign(4 * (size_t)nC * L.lpc, 0.0);

@@ -31,7 +31,8 @@ struct HostLayout {
     // per device cell: bits 0-7 forward dependency slots (interior neighbour
     // with a smaller mesh id), bits 8-15 backward slots (neighbour with a
     // larger mesh id, ghosts included), bits 16-23 backward slots whose
-    // neighbour is an interior cell
+    // neighbour is an interior cell, bits 24-31 boundary slots whose face
+    // value in the gradient is the cell/ghost average (not SYMM/INTERFACE)
     std::vector<int> depmask;
     // faces, in device order; face_of_dev[d] = mesh face id
     std::vector<int> face_of_dev;
@@ -75,6 +76,7 @@ struct HostLayout {
     std::vector<int> crun;                     // 4 per position
     std::vector<int> pos;                      // per chunk
     std::vector<int> snb;                      // lpc per device cell
+    std::vector<int> snode;                    // tetrahedral meshes: 3 face nodes per slot, 12 per cell
     std::vector<double> sgeo;                  // 4*lpc per device cell
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
