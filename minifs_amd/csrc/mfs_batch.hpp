@@ -41,6 +41,9 @@ struct BatchSolve {
     const typename Real<R>::vec4* fgeo_r;
     R *gx, *gy, *gz;
     const int* snode;             // tetrahedral meshes: slot nodes (nullptr otherwise)
+    const double4* sgeo64;        // FP64 slot geometry, distances and sides (residual kernel)
+    const double* sdn;
+    const int* sflip;
     // sweeps
     SweepArgs<R> sw;
 };
@@ -51,8 +54,9 @@ __global__ __launch_bounds__(kBlock) void k_b_flux_ts(const BatchSolve<R>* __res
     const BatchSolve<R>& S = T[cell_solve[blockIdx.x]];
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;  // XCD-aware inside the solve
-    flux_ts_body<MAXF, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.cfo, S.cf, S.fcell, S.fgeo, S.fdn, S.q, S.vol,
-                              S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K, S.cfl, S.cfl_min, S.q32, S.diag32);
+    flux_ts_slot_body<MAXF == 4 ? 4 : 8, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.sw.snb, S.sgeo64, S.sdn,
+                                                S.sflip, S.q, S.vol, S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K,
+                                                S.cfl, S.cfl_min, S.q32, S.diag32);
 }
 
 // residual norms, dt extrema and the history record of every solve (one

@@ -89,6 +89,8 @@ struct mfs_ctx {
     int* tickets = nullptr;  // ticket counters of the flat sweep schedule: forward, then backward
     int* gface = nullptr;    // nB: device face of each ghost cell nC+k
     int* snode = nullptr;    // tetrahedral meshes: 3 face nodes per slot, 12 per cell
+    double* sdn = nullptr;   // per slot: |(x_fc - x_cc).n| on this cell's side
+    int* sflip = nullptr;    // per cell: slots whose face has this cell second (bits 0-7), face count (8-11)
     int ghost_refresh = 0;   // SetGhostVariables after each update in mfs_step
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;

@@ -212,15 +212,16 @@ bool launch_timestep(mfs_ctx* ctx, bool with_diag, double cfl, double cfl_min) {
 template <int MAXF>
 bool launch_flux_ts_t(mfs_ctx* ctx, double cfl, double cfl_min) {
     Timed t(ctx, "flux_ts");
+    constexpr int LPC = MAXF == 4 ? 4 : 8;  // slots per cell of the slot tables
     if (ctx->precision == MFS_MIXED)
-        hipLaunchKernelGGL((dev::k_flux_ts<MAXF, true>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
-                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res,
+        hipLaunchKernelGGL((dev::k_flux_ts_slot<LPC, true>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream,
+                           ctx->nC, ctx->snb, ctx->sgeo, ctx->sdn, ctx->sflip, ctx->q[ctx->cur], ctx->vol, ctx->res,
                            ctx->dt, ctx->diag, ctx->part, ctx->K, cfl, cfl_min, ctx->q32, ctx->diag32);
     else
-        hipLaunchKernelGGL((dev::k_flux_ts<MAXF, false>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
-                           ctx->cfo, ctx->cf, ctx->fcell, ctx->fgeo, ctx->fdn, ctx->q[ctx->cur], ctx->vol, ctx->res,
+        hipLaunchKernelGGL((dev::k_flux_ts_slot<LPC, false>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream,
+                           ctx->nC, ctx->snb, ctx->sgeo, ctx->sdn, ctx->sflip, ctx->q[ctx->cur], ctx->vol, ctx->res,
                            ctx->dt, ctx->diag, ctx->part, ctx->K, cfl, cfl_min, nullptr, nullptr);
-    return check_launch(ctx, "k_flux_ts");
+    return check_launch(ctx, "k_flux_ts_slot");
 }
 bool launch_flux_ts(mfs_ctx* ctx, double cfl, double cfl_min) {
     return ctx->maxf == 4 ? launch_flux_ts_t<4>(ctx, cfl, cfl_min) : launch_flux_ts_t<6>(ctx, cfl, cfl_min);
@@ -578,6 +579,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
          dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets") &&
          dalloc(ctx, ctx->gface, (size_t)L.nB, "ghost faces");
     if (!L.snode.empty()) ok = ok && dalloc(ctx, ctx->snode, L.snode.size(), "slot nodes");
+    ok = ok && dalloc(ctx, ctx->sdn, L.sdn.size(), "slot distances") && dalloc(ctx, ctx->sflip, L.sflip.size(), "slot sides");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
         // resident workgroups of the sweep launch
@@ -626,7 +628,9 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              upload(ctx, ctx->sgeo, reinterpret_cast<const double4*>(L.sgeo.data()), L.snb.size(), "upload sgeo") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask") &&
              upload(ctx, ctx->gface, gf.data(), (size_t)L.nB, "upload gface") &&
-             (L.snode.empty() || upload(ctx, ctx->snode, L.snode.data(), L.snode.size(), "upload snode"));
+             (L.snode.empty() || upload(ctx, ctx->snode, L.snode.data(), L.snode.size(), "upload snode")) &&
+             upload(ctx, ctx->sdn, L.sdn.data(), L.sdn.size(), "upload sdn") &&
+             upload(ctx, ctx->sflip, L.sflip.data(), L.sflip.size(), "upload sflip");
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
         std::vector<float> f32;
@@ -1155,6 +1159,9 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     E.diag32 = c->diag32;
     E.K = c->K;
     E.snode = c->snode;
+    E.sgeo64 = c->sgeo;
+    E.sdn = c->sdn;
+    E.sflip = c->sflip;
     E.nlo = c->nlo;
     E.ell_off = c->ell_off;
     E.ell_ab = c->ell_ab;

@@ -348,6 +348,80 @@ __global__ __launch_bounds__(kBlock) void k_timestep(int nC, const int* __restri
 // iteration.  The cell's own state is loaded once and only the neighbour's
 // is gathered per face.  With MIXED the FP32 inputs of the implicit
 // operator (state snapshot, Diag) are written as well.
+// Per cell: the faces' fluxes added in slot order (the cell's ascending
+// face order), the time step from the smallest dn/eig, the CFL choice and
+// the diagonal.  NS slots, the first nf used; oth = the face's other cell,
+// g = the face's stored normal and area (c1 -> c2), first = this cell is c1.
+template <int NS, bool MIXED>
+__device__ __forceinline__ void flux_ts_cell(int c, const double (&qc)[5], int nf, const int (&oth)[NS],
+                                             const double4 (&g)[NS], const double (&dnv)[NS], const bool (&first)[NS],
+                                             const double* __restrict__ q, const double* __restrict__ vol,
+                                             double* __restrict__ res, double* __restrict__ dt,
+                                             double* __restrict__ diag, const FlowConst& K, double cfl, double cfl_min,
+                                             float* __restrict__ q32, float* __restrict__ diag32, double (&r)[5],
+                                             double& dtc) {
+    const double c2tmp = K.gam * (qc[4] + K.p_bar) / qc[0];
+    const double a = sqrt(c2tmp);
+    dtc = kBIGd;
+    int det = 1;
+    double eig[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double qo[5], ql[5], qr[5], F[5];
+        load5(q + (size_t)oth[s] * 5, qo);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            ql[j] = first[s] ? qc[j] : qo[j];
+            qr[j] = first[s] ? qo[j] : qc[j];
+        }
+        roe_flux<double>(ql, qr, g[s].x, g[s].y, g[s].z, g[s].w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
+        double vn = qc[1] * g[s].x + qc[2] * g[s].y + qc[3] * g[s].z;
+        vn = fabs(vn);
+        eig[s] = vn + a;
+        const double dp = fabs(qr[4] - ql[4]) / (qr[4] + ql[4] + K.p_bar + K.p_bar);
+        if (s < nf) {
+            if (first[s]) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) r[j] -= F[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) r[j] += F[j];
+            }
+            dtc = smin(dtc, dnv[s] / eig[s]);
+            if (dp > 0.1) det = 0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) res[(size_t)c * 5 + j] = r[j];
+    const double p = qc[4];
+    double cfl_tmp;
+    if (p > K.p_break)
+        cfl_tmp = cfl;
+    else if (p < K.p_min)
+        cfl_tmp = cfl_min;
+    else
+        cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
+    if (!det) cfl_tmp = cfl_min;
+    dtc *= cfl_tmp;
+    dt[c] = dtc;
+    double dg = vol[c] / dtc;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (s < nf) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
+    diag[c] = dg;
+    if (MIXED) {
+        diag32[c] = (float)dg;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) q32[(size_t)c * 5 + j] = (float)qc[j];
+    }
+}
+
+// Residual, local time step and LU-SGS diagonal in one pass over a cell's
+// faces, for the fused iteration: the arithmetic of k_flux<false> and
+// k_timestep<true>, which both read the state at the start of the
+// iteration.  The cell's own state is loaded once and only the neighbour's
+// is gathered per face.  With MIXED the FP32 inputs of the implicit
+// operator (state snapshot, Diag) are written as well.
 template <int MAXF, bool MIXED>
 __device__ __forceinline__ void flux_ts_body(int c, int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
                                              const int2* __restrict__ fcell, const double4* __restrict__ fgeo,
@@ -363,79 +437,90 @@ __device__ __forceinline__ void flux_ts_body(int c, int nC, const int* __restric
         load5(q + (size_t)c * 5, qc);
         int fs[MAXF];
         const int nf = cell_faces<MAXF>(cfo, cf, c, fs);
-        int2 cc[MAXF];
+        int oth[MAXF];
         double4 g[MAXF];
-        double2 d[MAXF];
+        double dnv[MAXF];
+        bool first[MAXF];
 #pragma unroll
         for (int s = 0; s < MAXF; ++s) {
-            cc[s] = fcell[fs[s]];
+            const int2 cc = fcell[fs[s]];
+            const double2 d = fdn[fs[s]];
             g[s] = fgeo[fs[s]];
-            d[s] = fdn[fs[s]];
+            first[s] = (cc.x == c);  // c is the face's c1
+            oth[s] = first[s] ? cc.y : cc.x;
+            dnv[s] = first[s] ? d.x : d.y;
         }
-        const double c2tmp = K.gam * (qc[4] + K.p_bar) / qc[0];
-        const double a = sqrt(c2tmp);
-        double dtc = kBIGd;
-        int det = 1;
-        double eig[MAXF];
-#pragma unroll
-        for (int s = 0; s < MAXF; ++s) {
-            const bool first = (cc[s].x == c);  // c is the face's c1
-            double qo[5], ql[5], qr[5], F[5];
-            load5(q + (size_t)(first ? cc[s].y : cc[s].x) * 5, qo);
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                ql[j] = first ? qc[j] : qo[j];
-                qr[j] = first ? qo[j] : qc[j];
-            }
-            roe_flux<double>(ql, qr, g[s].x, g[s].y, g[s].z, g[s].w, K.gamm1, K.p_bar, K.alf_l, kTINYd, F);
-            const double dn = first ? d[s].x : d[s].y;
-            double vn = qc[1] * g[s].x + qc[2] * g[s].y + qc[3] * g[s].z;
-            vn = fabs(vn);
-            eig[s] = vn + a;
-            const double dp = fabs(qr[4] - ql[4]) / (qr[4] + ql[4] + K.p_bar + K.p_bar);
-            if (s < nf) {
-                if (first) {
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) r[j] -= F[j];
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) r[j] += F[j];
-                }
-                dtc = smin(dtc, dn / eig[s]);
-                if (dp > 0.1) det = 0;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 5; ++j) res[(size_t)c * 5 + j] = r[j];
-        const double p = qc[4];
-        double cfl_tmp;
-        if (p > K.p_break)
-            cfl_tmp = cfl;
-        else if (p < K.p_min)
-            cfl_tmp = cfl_min;
-        else
-            cfl_tmp = (p - K.p_min) / (K.p_break - K.p_min) * (cfl - cfl_min) + cfl_min;
-        if (!det) cfl_tmp = cfl_min;
-        dtc *= cfl_tmp;
-        dt[c] = dtc;
+        double dtc;
+        flux_ts_cell<MAXF, MIXED>(c, qc, nf, oth, g, dnv, first, q, vol, res, dt, diag, K, cfl, cfl_min, q32, diag32,
+                                  r, dtc);
         vmin = dtc;
         vmax = dtc;
-        double dg = vol[c] / dtc;
-#pragma unroll
-        for (int s = 0; s < MAXF; ++s)
-            if (s < nf) dg += 0.5 * g[s].w * eig[s] * K.lhs_omga;
-        diag[c] = dg;
-        if (MIXED) {
-            diag32[c] = (float)dg;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) q32[(size_t)c * 5 + j] = (float)qc[j];
-        }
     }
     double s2[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
     block_sum<5>(s2, pblk);
     block_minmax(vmin, vmax, pblk + 5, pblk + 6);
+}
+
+// The same from the per-slot tables laid out by cell (neighbour, normal
+// pointing away from the cell with its area, this side's distance, and the
+// slots whose face has this cell second): contiguous reads, then the
+// neighbours' states.  The stored normal is the slot normal negated back
+// where the cell is the face's second cell, exactly.
+template <int LPC, bool MIXED>
+__device__ __forceinline__ void flux_ts_slot_body(int c, int nC, const int* __restrict__ snb,
+                                                  const double4* __restrict__ sgeo, const double* __restrict__ sdn,
+                                                  const int* __restrict__ sflip, const double* __restrict__ q,
+                                                  const double* __restrict__ vol, double* __restrict__ res,
+                                                  double* __restrict__ dt, double* __restrict__ diag,
+                                                  double* __restrict__ pblk, const FlowConst& K, double cfl,
+                                                  double cfl_min, float* __restrict__ q32,
+                                                  float* __restrict__ diag32) {
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double vmin = kBIGd, vmax = 0.0;
+    if (c < nC) {
+        double qc[5];
+        load5(q + (size_t)c * 5, qc);
+        const unsigned sf = (unsigned)sflip[c];
+        const int nf = (int)((sf >> 8) & 15u);
+        int oth[LPC];
+        double4 g[LPC];
+        double dnv[LPC];
+        bool first[LPC];
+#pragma unroll
+        for (int s = 0; s < LPC; ++s) {
+            const size_t i = (size_t)c * LPC + s;
+            const double4 v = sgeo[i];
+            first[s] = ((sf >> s) & 1u) == 0;
+            g[s] = first[s] ? v : make_double4(-v.x, -v.y, -v.z, v.w);
+            oth[s] = snb[i];
+            dnv[s] = sdn[i];
+        }
+        double dtc;
+        flux_ts_cell<LPC, MIXED>(c, qc, nf, oth, g, dnv, first, q, vol, res, dt, diag, K, cfl, cfl_min, q32, diag32,
+                                 r, dtc);
+        vmin = dtc;
+        vmax = dtc;
+    }
+    double s2[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
+    block_sum<5>(s2, pblk);
+    block_minmax(vmin, vmax, pblk + 5, pblk + 6);
+}
+
+template <int LPC, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_flux_ts_slot(int nC, const int* __restrict__ snb,
+                                                         const double4* __restrict__ sgeo,
+                                                         const double* __restrict__ sdn, const int* __restrict__ sflip,
+                                                         const double* __restrict__ q, const double* __restrict__ vol,
+                                                         double* __restrict__ res, double* __restrict__ dt,
+                                                         double* __restrict__ diag, double* __restrict__ part,
+                                                         FlowConst K, double cfl, double cfl_min,
+                                                         float* __restrict__ q32, float* __restrict__ diag32) {
+    flux_ts_slot_body<LPC, MIXED>(xcd_block() * kBlock + (int)threadIdx.x, nC, snb, sgeo, sdn, sflip, q, vol, res,
+                                  dt, diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
 }
 
 template <int MAXF, bool MIXED>

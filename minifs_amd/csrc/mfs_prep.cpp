@@ -473,6 +473,10 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
         }
         L.snb.resize((size_t)nC * L.lpc);
         L.sgeo.assign(4 * (size_t)nC * L.lpc, 0.0);
+        L.sdn.assign((size_t)nC * L.lpc, 0.0);
+        L.sflip.assign(nC, 0);
+        for (int d = 0; d < nC; ++d)
+            L.sflip[d] = (L.cfo[d + 1] - L.cfo[d]) << 8;
         for (int d = 0; d < nC; ++d)
             for (int sl = 0; sl < L.lpc; ++sl) {
                 const size_t i = (size_t)d * L.lpc + sl;
@@ -482,6 +486,8 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
                     continue;
                 }
                 const bool flip = (L.c2[f] == d);
+                if (flip) L.sflip[d] |= 1 << sl;
+                L.sdn[i] = flip ? L.dn2[f] : L.dn1[f];
                 L.snb[i] = flip ? L.c1[f] : L.c2[f];
                 for (int j = 0; j < 3; ++j) L.sgeo[4 * i + j] = flip ? -L.fgeo[4 * (size_t)f + j] : L.fgeo[4 * (size_t)f + j];
                 L.sgeo[4 * i + 3] = L.fgeo[4 * (size_t)f + 3];

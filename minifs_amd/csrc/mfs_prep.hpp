@@ -77,6 +77,11 @@ struct HostLayout {
     std::vector<int> pos;                      // per chunk
     std::vector<int> snb;                      // lpc per device cell
     std::vector<int> snode;                    // tetrahedral meshes: 3 face nodes per slot, 12 per cell
+    // per device cell: bits 0-7 slots whose face has this cell as its second
+    // cell (c2), bits 8-11 the number of faces; per slot the distance
+    // |(x_fc - x_cc).n| on this cell's side (0 when empty)
+    std::vector<int> sflip;
+    std::vector<double> sdn;
     std::vector<double> sgeo;                  // 4*lpc per device cell
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
