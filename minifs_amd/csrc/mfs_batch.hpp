@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_b_flux_ts(const BatchSolve<R>* __res
 // residual norms, dt extrema and the history record of every solve (one
 // workgroup per solve)
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_b_reduce(const BatchSolve<R>* __restrict__ T) {
+__global__ __launch_bounds__(kReduceBlock) void k_b_reduce(const BatchSolve<R>* __restrict__ T) {
     const BatchSolve<R>& S = T[blockIdx.x];
     reduce_body(S.nblk_cell, S.part, S.run, S.hist, S.hist_slot, 1 | 2 | 4, S.ratio_dtmax);
 }

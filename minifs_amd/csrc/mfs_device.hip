@@ -237,7 +237,7 @@ bool launch_mixed_inputs(mfs_ctx* ctx) {
 
 bool launch_reduce(mfs_ctx* ctx, int mode, int slot) {
     Timed t(ctx, "reduce");
-    hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(kBlock), 0, ctx->stream, (int)ctx->nblk_cell, ctx->part, ctx->run,
+    hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(dev::kReduceBlock), 0, ctx->stream, (int)ctx->nblk_cell, ctx->part, ctx->run,
                        ctx->hist, slot, mode, ctx->P.ratio_dtmax);
     return check_launch(ctx, "k_reduce");
 }
@@ -1273,7 +1273,7 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         const dev::BatchSolve<R>* Tb = Tf + ns;
         hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                            b->cell_solve);
-        hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(kBlock), 0, b->stream, Tf);
+        hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(dev::kReduceBlock), 0, b->stream, Tf);
         hipLaunchKernelGGL((dev::k_b_clamp<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                            b->cell_solve);
         if (!batch_ok(b, hipGetLastError(), "batch flux/time step")) return MFS_ERR_HIP;

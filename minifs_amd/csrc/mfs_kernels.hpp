@@ -31,9 +31,9 @@ struct Real<float> {
     static constexpr bits kSent = 0x7FA5D51Fu;
 };
 
-template <int NV>
+template <int NV, int NW = kBlock / 64>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
-    __shared__ double s[kBlock / 64][NV];
+    __shared__ double s[NW][NV];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1)
 #pragma unroll
@@ -53,8 +53,9 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
     }
 }
 
+template <int NW = kBlock / 64>
 __device__ __forceinline__ void block_minmax(double vmin, double vmax, double* out_min, double* out_max) {
-    __shared__ double smin_[kBlock / 64], smax_[kBlock / 64];
+    __shared__ double smin_[NW], smax_[NW];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         vmin = smin(vmin, __shfl_down(vmin, off, 64));
@@ -597,27 +598,35 @@ __global__ __launch_bounds__(kBlock) void k_mixed_inputs(int nC, const double* _
 // Reductions.  mode bits: 1 = residual norms -> run[3..7], 2 = dt extrema
 // (running max/min of compute_timestep.cpp:914-918, clamp flag), 4 = write
 // history record `slot` from run[].
+// One workgroup of kReduceBlock threads reads each block's partials once
+// (norms and extrema together) and sums them in a fixed tree.
+constexpr int kReduceBlock = 1024;
 __device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__ part, double* __restrict__ run,
                                             double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
-    if (mode & 1) {
-        double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < nblk; b += kBlock)
+    constexpr int NW = kReduceBlock / 64;
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double a = kBIGd, b = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += kReduceBlock) {
+        const double* r = part + (size_t)k * 8;
+        if (mode & 1) {
 #pragma unroll
-            for (int j = 0; j < 5; ++j) s[j] += part[(size_t)b * 8 + j];
+            for (int j = 0; j < 5; ++j) s[j] += r[j];
+        }
+        if (mode & 2) {
+            a = smin(a, r[5]);
+            b = smax(b, r[6]);
+        }
+    }
+    if (mode & 1) {
         __shared__ double tot[5];
-        block_sum<5>(s, tot);
+        block_sum<5, NW>(s, tot);
         __syncthreads();
         if (threadIdx.x == 0)
             for (int j = 0; j < 5; ++j) run[3 + j] = sqrt(tot[j]);
     }
     if (mode & 2) {
-        double a = kBIGd, b = 0.0;
-        for (int k = threadIdx.x; k < nblk; k += kBlock) {
-            a = smin(a, part[(size_t)k * 8 + 5]);
-            b = smax(b, part[(size_t)k * 8 + 6]);
-        }
         __shared__ double mn, mx;
-        block_minmax(a, b, &mn, &mx);
+        block_minmax<NW>(a, b, &mn, &mx);
         __syncthreads();
         if (threadIdx.x == 0) {
             const double dmax = smax(run[0], mx);
@@ -635,7 +644,7 @@ __device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__
         h[6] = run[1];
     }
 }
-__global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
+__global__ __launch_bounds__(kReduceBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
                                                    double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
     reduce_body(nblk, part, run, hist, slot, mode, ratio_dtmax);
 }
