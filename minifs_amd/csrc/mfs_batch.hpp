@@ -8,10 +8,12 @@
 // solve owns a whole number of XCD groups of 8 workgroups).  The per-cell arithmetic is the
 // single-solve kernels' own (the same device functions), so a solve gives
 // the same bits alone and in a batch.  The sweeps walk a list of (solve, u)
-// items, the u-th chunk in level order of every solve for u = 0, 1, ...,
-// handed out by ticket groups; dependencies never cross solves and each
-// solve's chunks keep their order, so the deadlock argument of the
-// single-solve sweep holds.
+// items that interleaves the solves' chunks, handed out by ticket groups.
+// The forward list aligns the solves at their last chunk and the backward
+// list at their first, so that the thin tails of all solves' level orders
+// run side by side.  Dependencies never cross solves and each solve's
+// chunks keep their order, so the deadlock argument of the single-solve
+// sweep holds.
 
 template <typename R>
 struct BatchSolve {

@@ -1084,16 +1084,23 @@ bool batch_rebuild_maps(mfs_batch* b) {
         cs.insert(cs.end(), (size_t)c->nblk_cell, (int)a);
         ns.insert(ns.end(), (size_t)nblk_xcd(c->nN), (int)a);
     }
-    // flat schedule across solves: the u-th chunk (in level order) of every
-    // solve, u = 0, 1, ...; each solve's chunks stay in its sweep order
+    // Flat schedule across solves; each solve's chunks stay in its sweep
+    // order.  A solve's level order ends in a long thin tail (the cells on
+    // its longest dependency chains, a few chunks per level).  The waves
+    // that take tail chunks wait a level's latency each, so the tails of all
+    // solves must run side by side: a tail meeting only other solves' wide
+    // levels holds the waves in flight and the list behind it.  The tail is
+    // last in the forward sweep and first in the backward sweep, so the
+    // forward list aligns the solves at their last chunk (solve a starts at
+    // step maxch - nchunk_a) and the backward list at their first.
     int maxch = 0;
     for (size_t a = 0; a < b->active.size(); ++a) maxch = std::max(maxch, b->slots[b->active[a]]->nchunk);
-    for (int u = 0; u < maxch; ++u)
-        for (size_t a = 0; a < b->active.size(); ++a)
-            if (u < b->slots[b->active[a]]->nchunk) {
-                itf.push_back(make_int2((int)a, u));
-                itb.push_back(make_int2((int)a, u));
-            }
+    for (int k = 0; k < maxch; ++k)
+        for (size_t a = 0; a < b->active.size(); ++a) {
+            const int nch = b->slots[b->active[a]]->nchunk;
+            if (k < nch) itb.push_back(make_int2((int)a, k));
+            if (k >= maxch - nch) itf.push_back(make_int2((int)a, k - (maxch - nch)));
+        }
     b->ncell_blk = (int)cs.size();
     b->nnode_blk = (int)ns.size();
     b->nitems = (int)itf.size();
