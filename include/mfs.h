@@ -243,6 +243,14 @@ int mfs_batch_destroy(mfs_batch* b);
  * removed.  mfs_batch_ctx(b, slot) gives its context for mfs_get_history,
  * mfs_download, mfs_get_scalars (do not destroy it directly). */
 int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_params* p, int* slot);
+/* Moves a context made by mfs_ctx_create(_ex), with the batch's precision
+ * and device and in no batch yet, into the batch, which owns it from then
+ * on (mfs_batch_add is create + attach).  Building a context is host work
+ * proportional to the mesh; it may run on another thread while the batch
+ * steps (the library keeps no shared mutable state), and attaching is
+ * cheap.  A solve stepped alone before attaching continues where it stood.
+ * On error the caller keeps the context. */
+int mfs_batch_attach(mfs_batch* b, mfs_ctx* ctx, int* slot);
 int mfs_batch_remove(mfs_batch* b, int slot);
 int mfs_batch_step(mfs_batch* b, int n); /* n iterations of every solve of the batch */
 int mfs_batch_sync(mfs_batch* b);

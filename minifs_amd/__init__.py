@@ -274,6 +274,17 @@ class Batch:
         self.members[slot.value] = m
         return m
 
+    def attach(self, solver: Solver) -> _BatchMember:
+        """Move a Solver's context (same precision, built on any thread,
+        possibly stepped alone already) into the batch; the Solver object is
+        emptied and the returned member stands for the solve."""
+        slot = C.c_int()
+        check(lib().mfs_batch_attach(self._h, solver._h, C.byref(slot)), "mfs_batch_attach")
+        solver._h = None
+        m = _BatchMember(self, slot.value, solver.grid, solver.params)
+        self.members[slot.value] = m
+        return m
+
     def remove(self, member: _BatchMember):
         check(lib().mfs_batch_remove(self._h, member.slot), "mfs_batch_remove")
         self.members.pop(member.slot, None)

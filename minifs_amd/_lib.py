@@ -111,6 +111,7 @@ _SIGS = {
     "mfs_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "mfs_batch_destroy": (C.c_int, [C.c_void_p]),
     "mfs_batch_add": (C.c_int, [C.c_void_p, C.POINTER(MfsMeshView), C.POINTER(MfsParams), C.POINTER(C.c_int)]),
+    "mfs_batch_attach": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
     "mfs_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
     "mfs_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "mfs_batch_sync": (C.c_int, [C.c_void_p]),

@@ -1378,42 +1378,45 @@ extern "C" int mfs_batch_create(int precision, mfs_batch** out) {
     return MFS_OK;
 }
 
-extern "C" int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_params* p, int* slot) {
-    if (!b || !mesh || !p || !slot) {
-        set_error("mfs_batch_add: null argument");
+extern "C" int mfs_batch_attach(mfs_batch* b, mfs_ctx* c, int* slot) {
+    if (!b || !c || !slot) {
+        set_error("mfs_batch_attach: null argument");
+        return MFS_ERR_ARG;
+    }
+    if (!c->owns_stream) {
+        set_error("mfs_batch_attach: the context already belongs to a batch");
+        return MFS_ERR_ARG;
+    }
+    if (c->precision != b->precision || c->device != b->device) {
+        set_error("mfs_batch_attach: the context's precision or device differs from the batch's");
         return MFS_ERR_ARG;
     }
     if (hipSetDevice(b->device) != hipSuccess) {
-        set_error("mfs_batch_add: hipSetDevice failed");
+        set_error("mfs_batch_attach: hipSetDevice failed");
         return MFS_ERR_HIP;
     }
-    mfs_ctx* c = nullptr;
-    int r = mfs_ctx_create(mesh, p, b->precision, &c);
-    if (r != MFS_OK) return r;
     const bool first = b->maxf == 0;
     if (!first && (c->maxf != b->maxf || c->lpc != b->lpc)) {
-        (void)mfs_ctx_destroy(c);
-        set_error("mfs_batch_add: a batch holds either tetrahedral meshes or meshes with up to 6 faces per cell, not both");
+        set_error("mfs_batch_attach: a batch holds either tetrahedral meshes or meshes with up to 6 faces per cell, not both");
         return MFS_ERR_MESH;
     }
     if (first) {
-        b->maxf = c->maxf;
-        b->lpc = c->lpc;
         int per_cu = 0, cus = 0;
         const bool mixed = b->precision == MFS_MIXED;
-        const int occ = b->lpc == 4 ? (mixed ? batch_sweep_occupancy<4, float>(&per_cu)
+        const int occ = c->lpc == 4 ? (mixed ? batch_sweep_occupancy<4, float>(&per_cu)
                                              : batch_sweep_occupancy<4, double>(&per_cu))
                                     : (mixed ? batch_sweep_occupancy<8, float>(&per_cu)
                                              : batch_sweep_occupancy<8, double>(&per_cu));
         if (occ != 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) {
-            (void)mfs_ctx_destroy(c);
-            set_error("mfs_batch_add: occupancy query failed");
+            set_error("mfs_batch_attach: occupancy query failed");
             return MFS_ERR_HIP;
         }
+        b->maxf = c->maxf;
+        b->lpc = c->lpc;
         b->sweep_grid = std::max(1, std::min(per_cu, kSweepWavesPerCU / kBatchSweepWaves) * cus);
     }
     // the solve's work goes to the batch stream from now on
-    (void)hipStreamSynchronize(c->stream);
+    if (!batch_ok(b, hipStreamSynchronize(c->stream), "mfs_batch_attach: context stream")) return MFS_ERR_HIP;
     (void)hipStreamDestroy(c->stream);
     c->stream = b->stream;
     c->owns_stream = false;
@@ -1431,6 +1434,23 @@ extern "C" int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_
     b->dirty = true;
     *slot = s;
     return MFS_OK;
+}
+
+extern "C" int mfs_batch_add(mfs_batch* b, const mfs_mesh_view* mesh, const mfs_params* p, int* slot) {
+    if (!b || !mesh || !p || !slot) {
+        set_error("mfs_batch_add: null argument");
+        return MFS_ERR_ARG;
+    }
+    if (hipSetDevice(b->device) != hipSuccess) {
+        set_error("mfs_batch_add: hipSetDevice failed");
+        return MFS_ERR_HIP;
+    }
+    mfs_ctx* c = nullptr;
+    int r = mfs_ctx_create(mesh, p, b->precision, &c);
+    if (r != MFS_OK) return r;
+    r = mfs_batch_attach(b, c, slot);
+    if (r != MFS_OK) (void)mfs_ctx_destroy(c);
+    return r;
 }
 
 extern "C" int mfs_batch_remove(mfs_batch* b, int slot) {

@@ -6,8 +6,9 @@ server overlaps three things:
 
 * host set-up (mesh generation, TDTree ordering, metrics: the C++ host code
   releases the GIL through ctypes) on a pool of worker threads;
-* admission of set-up solves into one device batch (mfs_batch_*) while the
-  device memory they need fits the budget;
+* admission of set-up solves while the device memory they need fits the
+  budget: their device contexts (host layout, upload) are built on worker
+  threads and then attached to the one device batch (mfs_batch_attach);
 * batched device iterations of every admitted solve (one launch per kernel
   for all of them), with convergence checked every `check_every` steps;
   converged solves leave the batch and free their memory for the next.
@@ -101,40 +102,52 @@ class Server:
     def run(self, requests: list[Request], progress=None) -> dict:
         reqs = sorted(requests, key=lambda r: r.arrival)
         batch = Batch(self.precision)
-        pool = cf.ThreadPoolExecutor(self.workers)
+        # two pools, so that building a device context (host layout and
+        # upload, once memory is reserved) never queues behind mesh set-ups
+        setup_pool = cf.ThreadPoolExecutor(self.workers)
+        ctx_pool = cf.ThreadPoolExecutor(max(1, self.workers // 2))
         start = time.perf_counter()
         now = lambda: time.perf_counter() - start  # noqa: E731
         nxt = 0                     # next request not yet submitted for set-up
         building: dict[cf.Future, Request] = {}
         ready: list[tuple[Request, Grid, object]] = []
+        creating: dict[cf.Future, tuple[Request, Grid, float]] = {}
         live: dict[int, tuple[Request, object, Grid]] = {}   # slot -> (request, member, grid)
-        used = 0.0
+        used = 0.0                  # device bytes reserved or held
         done = []
         peak_live = 0
         while len(done) < len(reqs):
             t = now()
             while nxt < len(reqs) and reqs[nxt].arrival <= t:
-                building[pool.submit(_setup, reqs[nxt], self.cfl)] = reqs[nxt]
+                building[setup_pool.submit(_setup, reqs[nxt], self.cfl)] = reqs[nxt]
                 nxt += 1
             for fut in [f for f in building if f.done()]:
                 r = building.pop(fut)
                 g, p = fut.result()
                 r.t_ready = now()
                 ready.append((r, g, p))
-            # admission in arrival order while the memory budget allows
+            ready.sort(key=lambda e: e[0].arrival)
+            # reserve memory in arrival order while the budget allows; the
+            # context is built on a worker while the batch steps
             while ready:
                 r, g, p = ready[0]
-                nc = g.info["nTCell"]
-                need = nc * self.bytes_per_cell
-                if live and used + need > self.budget:
+                need = g.info["nTCell"] * self.bytes_per_cell
+                if (live or creating) and used + need > self.budget:
                     break
                 ready.pop(0)
-                m = batch.add(g, p)
+                used += need
+                creating[ctx_pool.submit(Solver, g, p, self.precision)] = (r, g, need)
+            # built contexts join the batch (cheap: no host layout work here)
+            for fut in [f for f in creating if f.done()]:
+                r, g, need = creating.pop(fut)
+                s = fut.result()
+                nc = g.info["nTCell"]
+                dev_bytes = s.info()["device_bytes"]
+                self.bytes_per_cell = max(self.bytes_per_cell, dev_bytes / nc)
+                used += dev_bytes - need
+                m = batch.attach(s)
                 r.t_admit = now()
                 r.n_cells = nc
-                dev_bytes = m.info()["device_bytes"]
-                self.bytes_per_cell = max(self.bytes_per_cell, dev_bytes / nc)
-                used += dev_bytes
                 live[m.slot] = (r, m, g)
             peak_live = max(peak_live, len(live))
             if live:
@@ -155,14 +168,14 @@ class Server:
                         done.append(r)
                         if progress:
                             progress(r)
-            else:
-                # nothing on the device: wait for the next set-up or arrival
-                if building:
-                    cf.wait(list(building), timeout=0.05, return_when=cf.FIRST_COMPLETED)
-                elif nxt < len(reqs):
-                    time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+            elif building or creating:
+                # nothing on the device: wait for the next set-up or context
+                cf.wait(list(building) + list(creating), timeout=0.05, return_when=cf.FIRST_COMPLETED)
+            elif nxt < len(reqs):
+                time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
         wall = now()
-        pool.shutdown()
+        setup_pool.shutdown()
+        ctx_pool.shutdown()
         batch.close()
         return summarize(done, wall, peak_live)
 

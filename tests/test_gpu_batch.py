@@ -103,3 +103,44 @@ def test_batch_ghost_refresh_equals_standalone(precision):
     b.close()
     for g, _ in cases:
         g.close()
+
+
+@pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
+def test_attach_continues_standalone_and_threaded_contexts(precision):
+    """mfs_batch_attach: a context stepped alone and then attached continues
+    bit for bit, and one built on another thread while the batch steps joins
+    like mfs_batch_add."""
+    import concurrent.futures as cf
+
+    print(f"[batch] attach precision={precision}", flush=True)
+    a, c, t = _case(7, 0.6, 1.0, 31), _case(6, 0.75, 2.5, 32), _case(9, 0.4, -2.0, 33)
+    b = mfs.Batch(precision)
+    mc = b.add(*c)
+    s = mfs.Solver(*a, precision=precision)
+    s.step(3)
+    with cf.ThreadPoolExecutor(1) as pool:
+        fut = pool.submit(mfs.Solver, *t, precision=precision)
+        b.step(2)
+        st = fut.result()
+    ma = b.attach(s)
+    assert s._h is None
+    mt = b.attach(st)
+    assert len(b) == 3
+    b.step(4)
+    for m, case, iters in ((ma, a, 7), (mc, c, 6), (mt, t, 4)):
+        ref, href = _alone(*case, iters, precision)
+        _same(m, ref, href)
+    b.close()
+
+
+def test_attach_rejects_another_precision():
+    g, p = _case(4, 0.5, 0.0, 41)
+    b = mfs.Batch(mfs.MFS_MIXED)
+    s = mfs.Solver(g, p, precision=mfs.MFS_FP64)
+    with pytest.raises(mfs.MfsError):
+        b.attach(s)
+    s.step(2)  # still the caller's context
+    assert len(s.history()) == 2
+    s.close()
+    assert len(b) == 0
+    b.close()
