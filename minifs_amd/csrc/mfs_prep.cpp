@@ -446,13 +446,20 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
         // the sweeps' tables: chunk records by position, slot neighbours and
         // oriented geometry (the normal is negated where the cell is the
         // face's second cell, exactly as the sweep would negate it)
+        // A chunk of a thin level (at most kThinLevelChunks chunks) has its
+        // lanes poll their own dependencies at once instead of one lane
+        // polling the latest first: few waves poll there, and the second
+        // round trip is a level's latency.  Flag: bit 8 of the cell count.
+        std::vector<int> lw(L.nlev, 0);
+        for (int t = 0; t < nchunk; ++t) ++lw[L.flevel[L.cell_of_dev[L.chunk[2 * t]]]];
         L.crun.resize(4 * (size_t)nchunk);
         L.pos.resize(nchunk);
         for (int u = 0; u < nchunk; ++u) {
             const int t = L.run[u];
+            const bool thin = lw[L.flevel[L.cell_of_dev[L.chunk[2 * t]]]] <= kThinLevelChunks;
             L.pos[t] = u;
             L.crun[4 * (size_t)u] = L.chunk[2 * t];
-            L.crun[4 * (size_t)u + 1] = L.chunk[2 * t + 1];
+            L.crun[4 * (size_t)u + 1] = L.chunk[2 * t + 1] | (thin ? kThinChunkFlag : 0);
             L.crun[4 * (size_t)u + 2] = L.rep_f[t];
             L.crun[4 * (size_t)u + 3] = L.rep_b[t];
         }

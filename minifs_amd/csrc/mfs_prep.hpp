@@ -68,8 +68,8 @@ struct HostLayout {
     std::vector<int> rep_f, rep_b;             // per chunk: dependency expected to be published last (-1: none)
     std::vector<int> cfs;                      // lpc slots per device cell: device face id or -1
     // what the sweeps read, laid out so that a chunk needs no chain of
-    // lookups: per position u of `run`, {first device cell, cell count,
-    // rep_f, rep_b} of chunk run[u]; per chunk, its position; per device
+    // lookups: per position u of `run`, {first device cell, cell count
+    // (| kThinChunkFlag on a thin level), rep_f, rep_b} of chunk run[u]; per chunk, its position; per device
     // cell and slot, the neighbour across the slot's face (the cell itself
     // when the slot is empty) and the face's unit normal pointing away from
     // the cell with its area (zeros when empty)
@@ -89,6 +89,10 @@ struct HostLayout {
 
 // Cells per storage tile of the device order (the sweeps' workgroup unit).
 constexpr int kDefaultTileCells = 4096;
+// Sweep levels of at most this many chunks are thin: their chunks are
+// flagged in the count word of their record (crun[4u+1] bit 8).
+constexpr int kThinLevelChunks = 64;
+constexpr int kThinChunkFlag = 0x100;
 
 // Returns "" on success.
 std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int tile_cells = kDefaultTileCells);

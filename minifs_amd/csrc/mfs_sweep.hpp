@@ -41,7 +41,9 @@
 // device-coherent loads and retries while any is the sentinel (each store
 // is single-copy atomic, so a non-sentinel value is final).  One lane per
 // wave first waits on the chunk's latest dependency (rep), which keeps the
-// polling traffic to one load per wave.  The forward sweep re-arms the
+// polling traffic to one load per wave; on thin levels (few chunks, few
+// waves polling) every lane polls its own dependency at once, which saves
+// the second round trip on the level's critical path.  The forward sweep re-arms the
 // backward array and the backward sweep the forward one.
 //
 // Precision: R is the type of the implicit operator (state snapshot,
@@ -184,7 +186,8 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     const int rep = FWD ? rec.z : rec.w;
     B rv = 0;
     if (lane == 0 && rep >= 0) rv = ld_dev<R>(poll + (size_t)rep * 5);
-    const bool active = k < rec.y;
+    const bool active = k < (rec.y & 0xff);
+    const bool thin = (rec.y & 0x100) != 0;  // a thin level: every lane polls its own dependency
     const int c = rec.x + (active ? k : 0);  // idle lanes read the first cell and store nothing
     const unsigned m = (unsigned)A.depmask[c];
     unsigned kind = FWD ? (m & 0xffu) : ((m >> 8) & 0xffu);  // slots that take part
@@ -231,7 +234,7 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     }
 
     unsigned spins = 0;
-    if (!rep_seen) {  // one lane waits for the chunk's latest dependency
+    if (!rep_seen && !thin) {  // one lane waits for the chunk's latest dependency
         while (true) {
             B v = 0;
             if (lane == 0) v = ld_dev<R>(poll + (size_t)rep * 5);
