@@ -64,6 +64,11 @@ KERNEL_BYTES = {
 ITER_BYTES = (268, 840, 356, 92)
 
 
+# A shallower TDTree cell order than the reference's 4 parts / 512-cell
+# leaves: on the 100^3 x 6 mesh 82 sweep levels instead of 342.
+SHALLOW_TREE = (2, 128)
+
+
 def alg_bytes(coef, nF, nC, nB, nN):
     return coef[0] * nF + coef[1] * nC + coef[2] * nB + coef[3] * nN
 
@@ -191,7 +196,7 @@ def mode_fp64(args) -> dict:
 
     t0 = time.time()
     p = mfs.make_params(mach=args.mach, alpha=args.alpha, cfl=args.cfl)
-    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple().preprocess(p)
+    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple(*args.tree).preprocess(p)
     info = g.info
     s = mfs.Solver(g, p)
     setup_s = time.time() - t0
@@ -228,6 +233,21 @@ def mode_fp64(args) -> dict:
                  "kernel_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in kern_m.items()}}
         sm.close()
     g.close()
+    shallow = None
+    if not args.no_shallow and tuple(args.tree) != SHALLOW_TREE:
+        # the same mesh and flow under a shallower cell order (a set-up
+        # choice: any order is a valid input to the path)
+        g2 = mfs.Grid.box_bump(args.n, seed=args.seed).decouple(*SHALLOW_TREE).preprocess(p)
+        shallow = {"ordering": f"TDTree {SHALLOW_TREE[0]} parts, {SHALLOW_TREE[1]}-cell leaves"}
+        for prec, key in ((mfs.MFS_FP64, "fp64"), (mfs.MFS_MIXED, "mixed")):
+            s2 = mfs.Solver(g2, p, precision=prec)
+            if prec == mfs.MFS_FP64:
+                shallow["sweep_levels"] = s2.info()["sweep_levels"]
+            el2, _ = timed_steps(s2, args.steps, args.warmup, False)
+            shallow[key] = {"value": round(nC * args.steps / el2 / 1e6, 3), "unit": "M cell-iters/s",
+                            "ms_per_step": round(el2 / args.steps * 1e3, 4)}
+            s2.close()
+        g2.close()
     return {
         "metric": METRIC,
         "value": round(value, 3),
@@ -243,13 +263,15 @@ def mode_fp64(args) -> dict:
         "data": "synthetic",
         "config": {"workload": f"plain FP64 LU-SGS iteration, box+bump {args.n}^3x6 tets", "cells": nC,
                    "faces": nF, "bfaces": nB, "nodes": nN, "mach": args.mach, "alpha_deg": args.alpha,
-                   "cfl": args.cfl, "seed": args.seed, "ordering": "TDTree", "sweep_levels": info_ctx["sweep_levels"]},
+                   "cfl": args.cfl, "seed": args.seed, "ordering": f"TDTree {args.tree[0]} parts, {args.tree[1]}-cell leaves",
+                   "sweep_levels": info_ctx["sweep_levels"]},
         "roofline": roof,
         "iteration_roofline": {"alg_bytes_per_iter": int(iter_b),
                                "achieved_GBs": round(iter_b / (ms_per_step / 1e3) / 1e9, 2),
                                "frac": round(iter_b / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)},
         "kernel_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
         "mixed": mixed,
+        "shallow_order": shallow,
         "setup_s": round(setup_s, 2),
         "last_residual": [float(x) for x in hist[-1][:5]] if len(hist) else None,
         "cpu_baseline": None,
@@ -265,7 +287,7 @@ def mode_converge(args) -> dict:
     import minifs_amd as mfs
 
     p = mfs.make_params(mach=args.mach, alpha=args.alpha, cfl=args.cfl)
-    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple().preprocess(p)
+    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple(*args.tree).preprocess(p)
     nC = g.info["nTCell"]
     runs = {}
     for prec, name in ((mfs.MFS_FP64, "fp64"), (mfs.MFS_MIXED, "mixed")):
@@ -331,16 +353,18 @@ def mode_batch_converge(args) -> dict:
     prec = mfs.MFS_MIXED if args.precision == "mixed" else mfs.MFS_FP64
     batch_reqs = reqs()
     srv = Server(prec, hbm_budget=args.hbm_budget, drop=args.batch_drop, max_iters=args.max_iters,
-                 check_every=args.check_every, workers=min(16, os.cpu_count() or 4))
+                 check_every=args.check_every, workers=min(16, os.cpu_count() or 4), tree=args.tree)
     served = srv.run(batch_reqs, progress=lambda r: print(f"[batch] solve {r.rid} {r.n_cells} cells {r.iters} it "
                                                           f"{r.t_done:.2f} s", file=sys.stderr, flush=True))
     print(json.dumps({"batched": served}), file=sys.stderr, flush=True)
     out = {"metric": "batched solves to a residual drop", "mode": "batch", "batched": served,
            "config": {"solves": args.solves, "hex_per_axis": [args.batch_lo, args.batch_hi], "seed": args.batch_seed,
-                      "precision": args.precision, "drop": args.batch_drop, "check_every": args.check_every}}
+                      "precision": args.precision, "drop": args.batch_drop, "check_every": args.check_every,
+                      "tree": list(args.tree)}}
     if not args.no_sequential:
         seq_reqs = reqs()
         seq = run_sequential(seq_reqs, drop=args.batch_drop, max_iters=args.max_iters, check_every=args.check_every,
+                             tree=args.tree,
                              progress=lambda r: print(f"[sequential] solve {r.rid} {r.n_cells} cells {r.iters} it "
                                                       f"{r.t_done:.2f} s", file=sys.stderr, flush=True))
         out["sequential_fp64"] = seq
@@ -367,7 +391,7 @@ def mode_batch(args) -> dict:
     def build(sp):
         nx, ny, nz, bh, mach, alpha, seed = sp
         p = mfs.make_params(mach=mach, alpha=alpha, cfl=args.cfl)
-        return mfs.Grid.box_bump(nx, ny, nz, bump_height=bh, seed=seed).decouple().preprocess(p), p
+        return mfs.Grid.box_bump(nx, ny, nz, bump_height=bh, seed=seed).decouple(*args.tree).preprocess(p), p
 
     t0 = time.time()
     with cf.ThreadPoolExecutor(min(16, os.cpu_count() or 4)) as pool:
@@ -408,7 +432,7 @@ def mode_serve(args) -> dict:
     prec = mfs.MFS_MIXED if args.precision == "mixed" else mfs.MFS_FP64
     reqs = make_requests(args.requests, args.rate, args.cells_lo, args.cells_hi, args.serve_seed)
     srv = Server(prec, hbm_budget=args.hbm_budget, drop=args.serve_drop, max_iters=args.max_iters,
-                 check_every=args.check_every, workers=min(16, os.cpu_count() or 4))
+                 check_every=args.check_every, workers=min(16, os.cpu_count() or 4), tree=args.tree)
     served = srv.run(reqs, progress=lambda r: print(f"[serve] request {r.rid} {r.n_cells} cells {r.iters} it "
                                                     f"{r.latency:.2f} s", file=sys.stderr, flush=True))
     print(json.dumps({"served": served}), file=sys.stderr, flush=True)
@@ -416,6 +440,7 @@ def mode_serve(args) -> dict:
     if not args.no_sequential:
         again = make_requests(args.requests, args.rate, args.cells_lo, args.cells_hi, args.serve_seed)
         base = run_sequential(again, drop=args.serve_drop, max_iters=args.max_iters, check_every=args.check_every,
+                              tree=args.tree,
                               progress=lambda r: print(f"[sequential] request {r.rid} {r.n_cells} cells {r.iters} it "
                                                        f"{r.latency:.2f} s", file=sys.stderr, flush=True))
         same = all(a.rho_sum == b.rho_sum and a.iters == b.iters
@@ -427,7 +452,7 @@ def mode_serve(args) -> dict:
             "solves_per_hour_gain": round(served["solves_per_hour"] / base["solves_per_hour"], 3) if base else None,
             "config": {"requests": args.requests, "rate_per_s": args.rate, "cells": [args.cells_lo, args.cells_hi],
                        "drop": args.serve_drop, "precision": args.precision, "hbm_budget": args.hbm_budget,
-                       "seed": args.serve_seed}}
+                       "seed": args.serve_seed, "tree": list(args.tree)}}
 
 
 def main() -> int:
@@ -441,11 +466,14 @@ def main() -> int:
     ap.add_argument("--alpha", type=float, default=None, help="default 0 (1.25 for --mode converge)")
     ap.add_argument("--cfl", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--tree", type=lambda t: tuple(int(x) for x in t.split(",")), default=(4, 512),
+                    metavar="PARTS,LEAF", help="TDTree parameters of the mesh order (the reference's: 4,512)")
     ap.add_argument("--cpu-n", type=int, default=40, help="mesh of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--cpu-port-iters", type=int, default=60)
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision repeat of the timed region")
+    ap.add_argument("--no-shallow", action="store_true", help="skip the repeat on the shallower cell order")
     ap.add_argument("--drop", type=float, default=1e-10, help="converge mode: residual drop")
     ap.add_argument("--max-iters", type=int, default=20000)
     ap.add_argument("--check-every", type=int, default=50)

@@ -106,6 +106,14 @@ int mfs_grid_save(const mfs_grid* g, const char* mesh_path, const char* bc_path)
 /* Decoupling: TDTree decomposition and cell/node/face renumbering
  * (load_grid.cpp:339-395 under -DTDTREE).  Must precede preprocess. */
 int mfs_grid_decouple(mfs_grid* g);
+/* The same with the tree's parameters: nb_parts parts per tree node (the
+ * reference's CreateTree uses 4, grid_polyhedra.cpp:378-383) and leaves
+ * below part_size cells (512 there, :405).  The cell order sets the
+ * LU-SGS dependency depth: on the 100^3 x 6 box+bump mesh 4/512 gives 342
+ * sweep levels and 2/512 gives 129.  Any order is a valid input to the
+ * path; results match the reference run on the same renumbered mesh.
+ * Accepts 2 <= nb_parts <= 4 and part_size >= 8. */
+int mfs_grid_decouple_ex(mfs_grid* g, int nb_parts, int part_size);
 /* PreprocessGrid (load_grid.cpp:405-540): SpecifyBC, ComputeMetrics,
  * AllocAndInitFlowfield (freestream), SetGhostVariables. */
 int mfs_grid_preprocess(mfs_grid* g, const mfs_params* p);

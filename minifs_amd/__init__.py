@@ -70,8 +70,10 @@ class Grid:
     def save(self, mesh_path: str, bc_path: str) -> None:
         check(lib().mfs_grid_save(self._h, mesh_path.encode(), bc_path.encode()), "mfs_grid_save")
 
-    def decouple(self) -> "Grid":
-        check(lib().mfs_grid_decouple(self._h), "mfs_grid_decouple")
+    def decouple(self, nb_parts: int = 4, part_size: int = 512) -> "Grid":
+        """TDTree decomposition and renumbering (Decoupling); nb_parts and
+        part_size are the tree's parameters (the reference uses 4 and 512)."""
+        check(lib().mfs_grid_decouple_ex(self._h, nb_parts, part_size), "mfs_grid_decouple_ex")
         return self
 
     def preprocess(self, params: _lib.MfsParams) -> "Grid":

@@ -103,12 +103,18 @@ extern "C" int mfs_grid_save(const mfs_grid* g, const char* mesh_path, const cha
     return MFS_OK;
 }
 
-extern "C" int mfs_grid_decouple(mfs_grid* g) {
+extern "C" int mfs_grid_decouple(mfs_grid* g) { return mfs_grid_decouple_ex(g, 4, 512); }
+
+extern "C" int mfs_grid_decouple_ex(mfs_grid* g, int nb_parts, int part_size) {
     if (!g) {
         set_error("mfs_grid_decouple: null");
         return MFS_ERR_ARG;
     }
-    std::string e = g->g.decouple();
+    if (nb_parts < 2 || nb_parts > 4 || part_size < 8) {
+        set_error("mfs_grid_decouple_ex: need 2 <= nb_parts <= 4 and part_size >= 8");
+        return MFS_ERR_ARG;
+    }
+    std::string e = g->g.decouple(nb_parts, part_size);
     if (!e.empty()) {
         set_error("mfs_grid_decouple: " + e);
         return MFS_ERR_STATE;

@@ -62,10 +62,12 @@ void params_derive(mfs_params& p) {
     p.alf_n = p.epsa_r;
 }
 
-std::string Grid::decouple() {
+std::string Grid::decouple(int nbParts, int partSize) {
     if (preprocessed) return "Decoupling must run before PreprocessGrid";
     if (decoupled) return "";
-    tree = build_tdtree(m);
+    if (nbParts < 2 || nbParts > 4 || partSize < 8)
+        return "TDTree parameters out of range (2 <= nbParts <= 4, partSize >= 8)";
+    tree = build_tdtree(m, nbParts, partSize);
     m = renumber(m, tree);
     decoupled = true;
     return "";

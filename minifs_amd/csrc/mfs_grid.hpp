@@ -27,7 +27,7 @@ struct Grid {
     std::vector<double> rho, u, v, w, p;                // nC + nB
 
     // Set-up steps; return "" on success or an error message.
-    std::string decouple();
+    std::string decouple(int nbParts = 4, int partSize = 512);
     std::string preprocess(const mfs_params& prm);
 };
 

@@ -75,10 +75,10 @@ def make_requests(n: int, rate: float, cells_lo: float, cells_hi: float, seed: i
     return out
 
 
-def _setup(r: Request, cfl: float):
+def _setup(r: Request, cfl: float, tree: tuple[int, int] = (4, 512)):
     p = make_params(mach=r.mach, alpha=r.alpha, cfl=cfl)
     g = Grid.box_bump(r.nx, r.ny, r.nz, bump_height=r.bump_height, seed=r.seed, cells=r.cells)
-    g.decouple().preprocess(p)
+    g.decouple(*tree).preprocess(p)
     return g, p
 
 
@@ -88,7 +88,8 @@ def _converged(h: np.ndarray, drop: float) -> bool:
 
 class Server:
     def __init__(self, precision: int, hbm_budget: float, drop: float = 1e-8, max_iters: int = 20000,
-                 check_every: int = 20, cfl: float = 10.0, workers: int = 8, bytes_per_cell: float | None = None):
+                 check_every: int = 20, cfl: float = 10.0, workers: int = 8, bytes_per_cell: float | None = None,
+                 tree: tuple[int, int] = (4, 512)):
         self.precision = precision
         self.budget = hbm_budget
         self.drop = drop
@@ -96,6 +97,7 @@ class Server:
         self.check_every = check_every
         self.cfl = cfl
         self.workers = workers
+        self.tree = tuple(tree)     # TDTree parameters of the meshes' cell order
         # device bytes per cell, refined from the contexts actually created
         self.bytes_per_cell = bytes_per_cell or (620.0 if precision == MFS_FP64 else 900.0)
 
@@ -119,7 +121,7 @@ class Server:
         while len(done) < len(reqs):
             t = now()
             while nxt < len(reqs) and reqs[nxt].arrival <= t:
-                building[setup_pool.submit(_setup, reqs[nxt], self.cfl)] = reqs[nxt]
+                building[setup_pool.submit(_setup, reqs[nxt], self.cfl, self.tree)] = reqs[nxt]
                 nxt += 1
             for fut in [f for f in building if f.done()]:
                 r = building.pop(fut)
@@ -181,7 +183,8 @@ class Server:
 
 
 def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int = 20000, check_every: int = 20,
-                   cfl: float = 10.0, precision: int = MFS_FP64, progress=None) -> dict:
+                   cfl: float = 10.0, precision: int = MFS_FP64, progress=None,
+                   tree: tuple[int, int] = (4, 512)) -> dict:
     """The baseline: requests served one at a time in arrival order, each set
     up and solved alone."""
     reqs = sorted(requests, key=lambda r: r.arrival)
@@ -192,7 +195,7 @@ def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int =
         wait = r.arrival - now()
         if wait > 0:
             time.sleep(wait)
-        g, p = _setup(r, cfl)
+        g, p = _setup(r, cfl, tree)
         r.t_ready = r.t_admit = now()
         s = Solver(g, p, precision=precision)
         r.n_cells = g.info["nTCell"]

@@ -20,10 +20,10 @@ pytestmark = pytest.mark.gpu
 HIST_RTOL = 1e-12
 
 
-def _case(tmp_path, ref_driver, n, mach, alpha, cfl, iters, dumps, cells="tet"):
-    print(f"[parity] box N={n} {cells} mach={mach} alpha={alpha} cfl={cfl} iters={iters}", flush=True)
+def _case(tmp_path, ref_driver, n, mach, alpha, cfl, iters, dumps, cells="tet", tree=(4, 512)):
+    print(f"[parity] box N={n} {cells} mach={mach} alpha={alpha} cfl={cfl} iters={iters} tree={tree}", flush=True)
     p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
-    g = mfs.Grid.box_bump(n, cells=cells).decouple()
+    g = mfs.Grid.box_bump(n, cells=cells).decouple(*tree)
     mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
     g.save(mesh, bc)
     out = str(tmp_path / "ref")
@@ -60,6 +60,20 @@ def test_step_bitwise_vs_reference(tmp_path, ref_driver, n, iters):
     assert h.shape == hr.shape
     np.testing.assert_array_equal(h[:, 5:], hr[:, 5:])  # running dt extrema are exact
     np.testing.assert_allclose(h[:, :5], hr[:, :5], rtol=HIST_RTOL, atol=0)
+    s.close()
+
+
+@pytest.mark.parametrize("tree", [(2, 128), (3, 64)])
+def test_other_tree_orders_bitwise_vs_reference(tmp_path, ref_driver, tree):
+    """Any cell order is a valid input: a mesh decoupled with other tree
+    parameters (a shallower sweep DAG) still gives the reference's bits."""
+    p, g, ref, out = _case(tmp_path, ref_driver, 12, 0.8, 1.25, 10.0, 25, dumps=(25,), tree=tree)
+    s = mfs.Solver(g, p)
+    s.step(25)
+    _assert_state_equal(s, out, 25, g.info["nTCell"])
+    h = s.history()
+    np.testing.assert_array_equal(h[:, 5:], ref["history"][:, 5:])
+    np.testing.assert_allclose(h[:, :5], ref["history"][:, :5], rtol=HIST_RTOL, atol=0)
     s.close()
 
 

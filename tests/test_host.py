@@ -140,14 +140,16 @@ def _tree(g):
     return [dict(zip("fc lc ff lf fn ln leaf iso nparts isonode s0 s1 s2 s3".split(), map(int, r))) for r in t]
 
 
-def test_tdtree_renumbering_invariants():
+@pytest.mark.parametrize("nb_parts,part_size", [(4, 512), (2, 128), (3, 64)])
+def test_tdtree_renumbering_invariants(nb_parts, part_size):
     """Restated TDTree_creation (TDTree_creation.cpp:131-519): permutations
     are bijections, boundary faces stay first, parts precede their
     separator, cells of different parts never share a face, and a cell keeps
-    the relative order of its part."""
+    the relative order of its part; also with other tree parameters than the
+    reference's 4 parts and 512-cell leaves."""
     g0 = mfs.Grid.box_bump(12)
     f2c0 = g0.array("f2c").reshape(-1, 2)
-    g = mfs.Grid.box_bump(12).decouple()
+    g = mfs.Grid.box_bump(12).decouple(nb_parts, part_size)
     nC, nF, nB, nN = (g.info[k] for k in ("nTCell", "nTFace", "nBFace", "nTNode"))
     cp, fp, npm = g.array("cellPerm"), g.array("facePerm"), g.array("nodePerm")
     for perm, n in ((cp, nC), (fp, nF), (npm, nN)):
@@ -194,6 +196,37 @@ def test_tdtree_renumbering_invariants():
             orig = np.argsort(cp)[rng]
             assert (np.diff(orig) > 0).all()
     assert (covered == 1).all()
+
+
+def test_decouple_rejects_bad_tree_parameters():
+    for nb_parts, part_size in ((1, 512), (5, 512), (4, 4)):
+        g = mfs.Grid.box_bump(3)
+        with pytest.raises(mfs.MfsError):
+            g.decouple(nb_parts, part_size)
+        assert g.info["decoupled"] == 0
+        g.close()
+
+
+def test_fewer_tree_parts_give_a_shallower_sweep():
+    """The cell order sets the LU-SGS dependency depth (forward level of a
+    cell = 1 + the deepest lower-numbered face neighbour); two parts per tree
+    node instead of four give a shallower DAG on the box+bump mesh."""
+
+    def depth(g):
+        nC, nB = g.info["nTCell"], g.info["nBFace"]
+        f2c = g.array("f2c").reshape(-1, 2)[nB:]
+        lo, hi = f2c.min(axis=1), f2c.max(axis=1)
+        nbr = [[] for _ in range(nC)]
+        for a, b in zip(lo.tolist(), hi.tolist()):
+            nbr[b].append(a)
+        lev = [0] * nC
+        for c in range(nC):
+            lev[c] = 1 + max((lev[x] for x in nbr[c]), default=-1)
+        return max(lev) + 1
+
+    d4 = depth(mfs.Grid.box_bump(14).decouple())
+    d2 = depth(mfs.Grid.box_bump(14).decouple(2, 128))
+    assert d2 < d4
 
 
 def test_preprocess_matches_reference_geometry_bitwise(tmp_path):

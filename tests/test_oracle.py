@@ -86,12 +86,14 @@ def test_polyhedral_meshes_history_and_digests(case, threads):
         assert d == c["state_sha256"][f"it10_{nm}"], nm
 
 
-def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver):
-    """Where the reference build exists: a mesh not in the fixtures, other flow."""
+@pytest.mark.parametrize("tree", [(4, 512), (2, 16)])
+def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver, tree):
+    """Where the reference build exists: a mesh not in the fixtures, other
+    flow, and a cell order from other tree parameters."""
     import minifs_amd as mfs
     from tests.refrun import load_state, run_reference
 
-    g = mfs.Grid.box_bump(5, 3, 4, seed=7).decouple()
+    g = mfs.Grid.box_bump(5, 3, 4, seed=7).decouple(*tree)
     mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
     g.save(mesh, bc)
     ref = run_reference(ref_driver, mesh, bc, str(tmp_path / "r"), 0.7, -2.0, 5.0, 6, dumps=(6,))
