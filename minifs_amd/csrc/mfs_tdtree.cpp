@@ -8,9 +8,14 @@
 #include "mfs_tdtree.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <exception>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace mfs {
 
@@ -24,110 +29,178 @@ struct Csr {
     const int* row(int r) const { return val.data() + off[r]; }
 };
 
-class Builder {
-   public:
-    Builder(const RawMesh& m, int nbParts, int partSize) : m_(m), nbParts_(nbParts), partSize_(partSize) {
-        t_.nbParts = nbParts;
-        t_.partSize = partSize;
+// Subtrees of at least this many cells build their parts on other threads.
+constexpr int kParallelCells = 32768;
+
+// Per-thread marks of the recursion.  Each call takes a fresh stamp, so an
+// entry counts only when its stamp is the current call's and nothing needs
+// clearing between calls.
+struct Scratch {
+    std::vector<int> nodeStamp, nodeLocal;  // by node: local numbering of a call's nodes
+    std::vector<int> markStamp, markPart;   // by cell: parts claimed by neighbours
+    std::vector<int> pos, posStamp;         // by face or node: position inside a call's range
+    int stamp = 0;
+    Scratch(int nN, int nC) : nodeStamp(nN, -1), nodeLocal(nN, -1), markStamp(nC, -1), markPart(nC, 0) {}
+};
+
+// What the subtrees share: the mesh, the cell tables and the permutations
+// (each subtree writes only its own cell, face and node ranges).
+struct Shared {
+    const RawMesh& m;
+    int nbParts, partSize;
+    TDTree t;
+    std::vector<int> f2nOff;
+    Csr c2n, c2f, c2c;
+    std::atomic<int> spare{0};  // threads that may still be started
+    std::mutex pool_mu;
+    std::vector<std::unique_ptr<Scratch>> pool;  // scratch of finished workers, reused
+
+    std::unique_ptr<Scratch> get_scratch() {
+        {
+            std::lock_guard<std::mutex> lk(pool_mu);
+            if (!pool.empty()) {
+                std::unique_ptr<Scratch> sc = std::move(pool.back());
+                pool.pop_back();
+                return sc;
+            }
+        }
+        return std::make_unique<Scratch>(m.nN, m.nC);
+    }
+    void put_scratch(std::unique_ptr<Scratch> sc) {
+        std::lock_guard<std::mutex> lk(pool_mu);
+        pool.push_back(std::move(sc));
+    }
+
+    Shared(const RawMesh& m_, int nbParts_, int partSize_) : m(m_), nbParts(nbParts_), partSize(partSize_) {
+        t.nbParts = nbParts_;
+        t.partSize = partSize_;
         const int nC = m.nC, nF = m.nF, nN = m.nN;
-        f2nOff_ = m.f2n_offsets();
-        // c2n / c2f / c2c as unique ascending lists (CreateTree, grid_polyhedra.cpp:376-525)
-        std::vector<std::vector<int>> c2n(nC), c2f(nC), c2c(nC);
+        f2nOff = m.f2n_offsets();
+        // c2n / c2f / c2c as unique ascending lists (CreateTree,
+        // grid_polyhedra.cpp:376-525): rows counted, filled in face order,
+        // then each row sorted and deduplicated in place
+        std::vector<int> nn(nC, 0), nf(nC, 0), ncc(nC, 0);
+        for (int i = 0; i < nF; ++i) {
+            const int c1 = m.f2c[2 * i], c2 = m.f2c[2 * i + 1];
+            const int k = f2nOff[i + 1] - f2nOff[i];
+            nn[c1] += k;
+            ++nf[c1];
+            if (i >= m.nB) {
+                nn[c2] += k;
+                ++nf[c2];
+                ncc[c1] += 2;
+                ncc[c2] += 2;
+            }
+        }
+        auto open_rows = [nC](const std::vector<int>& cnt, Csr& out) {
+            out.off.assign(nC + 1, 0);
+            for (int c = 0; c < nC; ++c) out.off[c + 1] = out.off[c] + cnt[c];
+            out.val.resize(out.off[nC]);
+        };
+        open_rows(nn, c2n);
+        open_rows(nf, c2f);
+        open_rows(ncc, c2c);
+        for (int c = 0; c < nC; ++c) nn[c] = c2n.off[c], nf[c] = c2f.off[c], ncc[c] = c2c.off[c];
         for (int i = 0; i < nF; ++i) {
             const int c1 = m.f2c[2 * i], c2 = m.f2c[2 * i + 1];
             const bool interior = (i >= m.nB);
-            for (int k = f2nOff_[i]; k < f2nOff_[i + 1]; ++k) {
-                c2n[c1].push_back(m.f2n[k]);
-                if (interior) c2n[c2].push_back(m.f2n[k]);
+            for (int k = f2nOff[i]; k < f2nOff[i + 1]; ++k) {
+                c2n.val[nn[c1]++] = m.f2n[k];
+                if (interior) c2n.val[nn[c2]++] = m.f2n[k];
             }
-            c2f[c1].push_back(i);
+            c2f.val[nf[c1]++] = i;
             if (interior) {
-                c2f[c2].push_back(i);
-                c2c[c1].push_back(c1);
-                c2c[c2].push_back(c2);
-                c2c[c1].push_back(c2);
-                c2c[c2].push_back(c1);
+                c2f.val[nf[c2]++] = i;
+                c2c.val[ncc[c1]++] = c1;
+                c2c.val[ncc[c1]++] = c2;
+                c2c.val[ncc[c2]++] = c2;
+                c2c.val[ncc[c2]++] = c1;
             }
         }
-        auto pack = [nC](std::vector<std::vector<int>>& rows, Csr& out) {
-            out.off.assign(nC + 1, 0);
+        auto close_rows = [nC](Csr& t) {
+            int w = 0;
             for (int c = 0; c < nC; ++c) {
-                std::sort(rows[c].begin(), rows[c].end());
-                rows[c].erase(std::unique(rows[c].begin(), rows[c].end()), rows[c].end());
-                out.off[c + 1] = out.off[c] + (int)rows[c].size();
+                int* const b = t.val.data() + t.off[c];
+                int* const e = t.val.data() + t.off[c + 1];
+                std::sort(b, e);
+                int* const u = std::unique(b, e);
+                t.off[c] = w;
+                for (int* q = b; q < u; ++q) t.val[w++] = *q;
             }
-            out.val.resize(out.off[nC]);
-            for (int c = 0; c < nC; ++c) std::copy(rows[c].begin(), rows[c].end(), out.val.begin() + out.off[c]);
-            std::vector<std::vector<int>>().swap(rows);
+            t.off[nC] = w;
+            t.val.resize(w);
+            t.val.shrink_to_fit();
         };
-        pack(c2n, c2n_);
-        pack(c2f, c2f_);
-        pack(c2c, c2c_);
+        close_rows(c2n);
+        close_rows(c2f);
+        close_rows(c2c);
 
-        t_.cellRev.resize(nC);
-        t_.cellPerm.resize(nC);
-        t_.faceRev.resize(nF);
-        t_.facePerm.resize(nF);
-        t_.nodeRev.resize(nN);
-        t_.nodePerm.resize(nN);
-        std::iota(t_.cellRev.begin(), t_.cellRev.end(), 0);
-        std::iota(t_.cellPerm.begin(), t_.cellPerm.end(), 0);
-        std::iota(t_.faceRev.begin(), t_.faceRev.end(), 0);
-        std::iota(t_.facePerm.begin(), t_.facePerm.end(), 0);
-        std::iota(t_.nodeRev.begin(), t_.nodeRev.end(), 0);
-        std::iota(t_.nodePerm.begin(), t_.nodePerm.end(), 0);
+        t.cellRev.resize(nC);
+        t.cellPerm.resize(nC);
+        t.faceRev.resize(nF);
+        t.facePerm.resize(nF);
+        t.nodeRev.resize(nN);
+        t.nodePerm.resize(nN);
+        std::iota(t.cellRev.begin(), t.cellRev.end(), 0);
+        std::iota(t.cellPerm.begin(), t.cellPerm.end(), 0);
+        std::iota(t.faceRev.begin(), t.faceRev.end(), 0);
+        std::iota(t.facePerm.begin(), t.facePerm.end(), 0);
+        std::iota(t.nodeRev.begin(), t.nodeRev.end(), 0);
+        std::iota(t.nodePerm.begin(), t.nodePerm.end(), 0);
 
-        nodeStamp_.assign(nN, -1);
-        nodeLocal_.assign(nN, -1);
-        markStamp_.assign(nC, -1);
-        markPart_.assign(nC, 0);
+            const unsigned hw = std::thread::hardware_concurrency();
+        spare = (int)std::min(8u, std::max(1u, hw)) - 1;
     }
+    bool take_thread() {
+        int v = spare.load();
+        while (v > 0)
+            if (spare.compare_exchange_weak(v, v - 1)) return true;
+        return false;
+    }
+};
 
-    TDTree run() {
-        const int nC = m_.nC, nF = m_.nF, nN = m_.nN, nB = m_.nB;
+class Worker {
+   public:
+    Worker(Shared& s, std::vector<TDTreeNode>* nodes) : S(s), N(nodes), own_(s.get_scratch()), sc_(*own_) {}
+    ~Worker() { S.put_scratch(std::move(own_)); }
+
+    void run_root() {
+        const int nC = S.m.nC, nF = S.m.nF, nN = S.m.nN, nB = S.m.nB;
         const int nPFace = nB;  // serial meshes: no interface faces
         // TDTree_creation (TDTree_creation.cpp:407-519)
         int root = new_node();
         init_node(root, 0, nC - 1, 0, nF - 1, 0, nN - 1, false);
-        t_.nodes[root].isLeaf = false;
-        t_.nodes[root].nbParts = 1;
-        t_.nodes[root].son.assign(1, -1);
+        (*N)[root].isLeaf = false;
+        (*N)[root].nbParts = 1;
+        (*N)[root].son.assign(1, -1);
         if (nPFace > 0) {
             int s = new_node();
-            t_.nodes[root].son[0] = s;
+            (*N)[root].son[0] = s;
             create_boundary(s, 0, nPFace - 1, 0, -1, false);
         }
         int iso = new_node();
-        t_.nodes[root].iso = iso;
+        (*N)[root].iso = iso;
         init_node(iso, 0, nC - 1, nPFace, nF - 1, 0, nN - 1, false);
-        t_.nodes[iso].isLeaf = false;
-        t_.nodes[iso].nbParts = 1;
-        t_.nodes[iso].son.assign(1, -1);  // no interface faces in a serial mesh
+        (*N)[iso].isLeaf = false;
+        (*N)[iso].nbParts = 1;
+        (*N)[iso].son.assign(1, -1);  // no interface faces in a serial mesh
         int cells = new_node();
-        t_.nodes[iso].iso = cells;
+        (*N)[iso].iso = cells;
         create_normal(cells, 0, nC - 1, nB, nF - 1, 0, nN - 1, false);
-        for (int i = 0; i < nC; ++i) t_.cellPerm[t_.cellRev[i]] = i;
-        for (int i = 0; i < nF; ++i) t_.facePerm[t_.faceRev[i]] = i;
-        for (int i = 0; i < nN; ++i) t_.nodePerm[t_.nodeRev[i]] = i;
-        return std::move(t_);
     }
 
    private:
-    const RawMesh& m_;
-    int nbParts_, partSize_;
-    TDTree t_;
-    std::vector<int> f2nOff_;
-    Csr c2n_, c2f_, c2c_;
-    std::vector<int> nodeStamp_, nodeLocal_;
-    std::vector<int> markStamp_, markPart_;
-    int stamp_ = 0;
+    Shared& S;
+    std::vector<TDTreeNode>* N;  // the node list this worker appends to
+    std::unique_ptr<Scratch> own_;
+    Scratch& sc_;
 
     int new_node() {
-        t_.nodes.emplace_back();
-        return (int)t_.nodes.size() - 1;
+        (*N).emplace_back();
+        return (int)(*N).size() - 1;
     }
     void init_node(int n, int fc, int lc, int ff, int lf, int fn, int ln, bool isIso) {
-        TDTreeNode& d = t_.nodes[n];
+        TDTreeNode& d = (*N)[n];
         d.firstCell = fc;
         d.lastCell = lc;
         d.firstFace = ff;
@@ -143,13 +216,13 @@ class Builder {
     // (stands in for METIS_PartGraphRecursive on the nodal graph).
     void rcb(std::vector<int>& ids, int lo, int hi, int part0, int nparts, std::vector<int>& partOfLocal) {
         if (nparts <= 1 || hi - lo <= 1) {
-            for (int k = lo; k < hi; ++k) partOfLocal[nodeLocal_[ids[k]]] = part0;
+            for (int k = lo; k < hi; ++k) partOfLocal[sc_.nodeLocal[ids[k]]] = part0;
             return;
         }
         double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
         for (int k = lo; k < hi; ++k) {
             const int n = ids[k];
-            const double c[3] = {m_.x[n], m_.y[n], m_.z[n]};
+            const double c[3] = {S.m.x[n], S.m.y[n], S.m.z[n]};
             for (int a = 0; a < 3; ++a) {
                 mn[a] = std::min(mn[a], c[a]);
                 mx[a] = std::max(mx[a], c[a]);
@@ -158,7 +231,7 @@ class Builder {
         int axis = 0;
         for (int a = 1; a < 3; ++a)
             if (mx[a] - mn[a] > mx[axis] - mn[axis]) axis = a;
-        const std::vector<double>& coord = axis == 0 ? m_.x : (axis == 1 ? m_.y : m_.z);
+        const std::vector<double>& coord = axis == 0 ? S.m.x : (axis == 1 ? S.m.y : S.m.z);
         const int nleft = nparts / 2;
         const int mid = lo + (int)(((long long)(hi - lo) * nleft) / nparts);
         std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](int a, int b) {
@@ -171,21 +244,21 @@ class Builder {
 
     // Local node numbering by first appearance (create_local_i2n,
     // TDTree_partitioning.cpp:43-58) and partition of those nodes.
-    // Returns the number of local nodes; nodeLocal_ holds the local ids.
+    // Returns the number of local nodes; sc_.nodeLocal holds the local ids.
     int partition_items(const std::vector<const int*>& rows, const std::vector<int>& lens, std::vector<int>& partOfLocal) {
-        ++stamp_;
+        ++sc_.stamp;
         std::vector<int> ids;
         for (size_t r = 0; r < rows.size(); ++r)
             for (int k = 0; k < lens[r]; ++k) {
                 const int n = rows[r][k];
-                if (nodeStamp_[n] != stamp_) {
-                    nodeStamp_[n] = stamp_;
-                    nodeLocal_[n] = (int)ids.size();
+                if (sc_.nodeStamp[n] != sc_.stamp) {
+                    sc_.nodeStamp[n] = sc_.stamp;
+                    sc_.nodeLocal[n] = (int)ids.size();
                     ids.push_back(n);
                 }
             }
         partOfLocal.assign(ids.size(), 0);
-        rcb(ids, 0, (int)ids.size(), 0, nbParts_, partOfLocal);
+        rcb(ids, 0, (int)ids.size(), 0, S.nbParts, partOfLocal);
         return (int)ids.size();
     }
 
@@ -194,7 +267,7 @@ class Builder {
     int item_part(const int* row, int len, const std::vector<int>& partOfLocal) {
         int a = -1, b = -1;
         for (int k = 0; k < len; ++k) {
-            const int p = partOfLocal[nodeLocal_[row[k]]];
+            const int p = partOfLocal[sc_.nodeLocal[row[k]]];
             if (a == -1)
                 a = p;
             else if (a != p)
@@ -224,29 +297,29 @@ class Builder {
         itemPart.assign(std::max(nItem, 0), ISO);
         if (nItem <= 0) return;
         // position lookup restricted to the range
-        ++stamp_;
-        std::vector<int>& pos = scratchPos_;
-        if ((int)pos.size() < (int)itemRev.size()) pos.assign(itemRev.size(), -1), posStamp_.assign(itemRev.size(), -1);
+        ++sc_.stamp;
+        std::vector<int>& pos = sc_.pos;
+        if ((int)pos.size() < (int)itemRev.size()) pos.assign(itemRev.size(), -1), sc_.posStamp.assign(itemRev.size(), -1);
         for (int k = fi; k <= li; ++k) {
             pos[itemRev[k]] = k - fi;
-            posStamp_[itemRev[k]] = stamp_;
+            sc_.posStamp[itemRev[k]] = sc_.stamp;
         }
         const int nCell = (int)cellPart.size();
         for (int i = 0; i < nCell; ++i) {
             if (cellPart[i] == ISO) continue;
-            const int c = t_.cellRev[fc + i];
+            const int c = S.t.cellRev[fc + i];
             for (int k = c2i.off[c]; k < c2i.off[c + 1]; ++k) {
                 const int it = c2i.val[k];
-                if (posStamp_[it] != stamp_) continue;
+                if (sc_.posStamp[it] != sc_.stamp) continue;
                 itemPart[pos[it]] = cellPart[i];
             }
         }
         for (int i = 0; i < nCell; ++i) {
             if (cellPart[i] != ISO) continue;
-            const int c = t_.cellRev[fc + i];
+            const int c = S.t.cellRev[fc + i];
             for (int k = c2i.off[c]; k < c2i.off[c + 1]; ++k) {
                 const int it = c2i.val[k];
-                if (posStamp_[it] != stamp_) continue;
+                if (sc_.posStamp[it] != sc_.stamp) continue;
                 itemPart[pos[it]] = ISO;
             }
         }
@@ -257,25 +330,24 @@ class Builder {
                 ++cnt[itemPart[k]];
         }
     }
-    std::vector<int> scratchPos_, posStamp_;
 
     // TDTree::TDTree_create_boundary (TDTree_creation.cpp:131-262)
     void create_boundary(int tree, int ff, int lf, int fn, int ln, bool isIso) {
         const int nFace = lf - ff + 1;
-        if (nFace < partSize_) {
+        if (nFace < S.partSize) {
             init_node(tree, 0, -1, ff, lf, fn, ln, isIso);
             return;
         }
         std::vector<const int*> rows(nFace);
         std::vector<int> lens(nFace);
         for (int i = 0; i < nFace; ++i) {
-            const int f = t_.faceRev[ff + i];
-            rows[i] = m_.f2n.data() + f2nOff_[f];
-            lens[i] = f2nOff_[f + 1] - f2nOff_[f];
+            const int f = S.t.faceRev[ff + i];
+            rows[i] = S.m.f2n.data() + S.f2nOff[f];
+            lens[i] = S.f2nOff[f + 1] - S.f2nOff[f];
         }
         std::vector<int> partOfLocal;
         partition_items(rows, lens, partOfLocal);
-        const int parts = nbParts_;
+        const int parts = S.nbParts;
         std::vector<int> facePart(nFace), nbPartFace(parts, 0);
         int nbIsoFace = 0;
         for (int i = 0; i < nFace; ++i) {
@@ -286,13 +358,13 @@ class Builder {
                 ++nbPartFace[facePart[i]];
         }
         // faces of one owner cell stay in one part (TDTree_creation.cpp:160-175)
-        ++stamp_;
+        ++sc_.stamp;
         for (int i = 0; i < nFace; ++i) {
             if (facePart[i] == ISO) continue;
-            const int c1 = m_.f2c[2 * t_.faceRev[ff + i]];
-            if (markStamp_[c1] != stamp_ || markPart_[c1] == facePart[i]) {
-                markStamp_[c1] = stamp_;
-                markPart_[c1] = facePart[i];
+            const int c1 = S.m.f2c[2 * S.t.faceRev[ff + i]];
+            if (sc_.markStamp[c1] != sc_.stamp || sc_.markPart[c1] == facePart[i]) {
+                sc_.markStamp[c1] = sc_.stamp;
+                sc_.markPart[c1] = facePart[i];
             } else {
                 --nbPartFace[facePart[i]];
                 facePart[i] = ISO;
@@ -302,22 +374,22 @@ class Builder {
         std::vector<int> nbPartNode(parts, 0);
         int nbIsoNode = 0;
         if (ln - fn + 1 > 0) throw std::logic_error("boundary face tree with a node range");
-        stable_by_part(t_.faceRev, ff, facePart);
+        stable_by_part(S.t.faceRev, ff, facePart);
 
         const int nbIsoItem = std::max(nbIsoFace, nbIsoNode);
         std::vector<int> nbPartItem(parts);
         for (int i = 0; i < parts; ++i) nbPartItem[i] = std::max(nbPartFace[i], nbPartNode[i]);
         init_node(tree, 0, -1, ff, lf, fn, ln, isIso);
-        t_.nodes[tree].nbParts = parts;
-        t_.nodes[tree].son.assign(parts, -1);
+        (*N)[tree].nbParts = parts;
+        (*N)[tree].son.assign(parts, -1);
         for (int i = 0; i < parts; ++i)
             if (nbPartItem[i] > 0) {
                 int s = new_node();
-                t_.nodes[tree].son[i] = s;
+                (*N)[tree].son[i] = s;
             }
         if (nbIsoItem > 0) {
             int s = new_node();
-            t_.nodes[tree].iso = s;
+            (*N)[tree].iso = s;
         }
         bool leaf = true;
         int st2 = ff;
@@ -325,31 +397,31 @@ class Builder {
             const int ed2 = st2 + nbPartFace[i] - 1;
             if (nbPartItem[i] > 0) {
                 leaf = false;
-                create_boundary(t_.nodes[tree].son[i], st2, ed2, 0, -1, isIso);
+                create_boundary((*N)[tree].son[i], st2, ed2, 0, -1, isIso);
             }
             st2 = ed2 + 1;
         }
-        t_.nodes[tree].isLeaf = leaf;
-        if (nbIsoItem > 0 && !leaf) create_boundary(t_.nodes[tree].iso, lf - nbIsoFace + 1, lf, 0, -1, true);
+        (*N)[tree].isLeaf = leaf;
+        if (nbIsoItem > 0 && !leaf) create_boundary((*N)[tree].iso, lf - nbIsoFace + 1, lf, 0, -1, true);
     }
 
     // TDTree::TDTree_create_normal (TDTree_creation.cpp:264-405)
     void create_normal(int tree, int fc, int lc, int ff, int lf, int fn, int ln, bool isIso) {
         const int nCell = lc - fc + 1;
-        if (nCell < partSize_) {
+        if (nCell < S.partSize) {
             init_node(tree, fc, lc, ff, lf, fn, ln, isIso);
             return;
         }
         std::vector<const int*> rows(nCell);
         std::vector<int> lens(nCell);
         for (int i = 0; i < nCell; ++i) {
-            const int c = t_.cellRev[fc + i];
-            rows[i] = c2n_.row(c);
-            lens[i] = c2n_.row_size(c);
+            const int c = S.t.cellRev[fc + i];
+            rows[i] = S.c2n.row(c);
+            lens[i] = S.c2n.row_size(c);
         }
         std::vector<int> partOfLocal;
         partition_items(rows, lens, partOfLocal);
-        const int parts = nbParts_;
+        const int parts = S.nbParts;
         std::vector<int> cellPart(nCell), nbPartCell(parts, 0);
         int nbIsoCell = 0;
         for (int i = 0; i < nCell; ++i) {
@@ -361,15 +433,15 @@ class Builder {
         }
         // a cell whose neighbourhood was claimed by another part becomes a
         // separator cell (TDTree_creation.cpp:293-313)
-        ++stamp_;
+        ++sc_.stamp;
         for (int i = 0; i < nCell; ++i) {
             if (cellPart[i] == ISO) continue;
-            const int c = t_.cellRev[fc + i];
-            const int* nb = c2c_.row(c);
-            const int nn = c2c_.row_size(c);
+            const int c = S.t.cellRev[fc + i];
+            const int* nb = S.c2c.row(c);
+            const int nn = S.c2c.row_size(c);
             for (int j = 0; j < nn; ++j) {
                 const int x = nb[j];
-                if (markStamp_[x] == stamp_ && markPart_[x] != cellPart[i]) {
+                if (sc_.markStamp[x] == sc_.stamp && sc_.markPart[x] != cellPart[i]) {
                     --nbPartCell[cellPart[i]];
                     ++nbIsoCell;
                     cellPart[i] = ISO;
@@ -378,52 +450,106 @@ class Builder {
             }
             if (cellPart[i] == ISO) continue;
             for (int j = 0; j < nn; ++j) {
-                markStamp_[nb[j]] = stamp_;
-                markPart_[nb[j]] = cellPart[i];
+                sc_.markStamp[nb[j]] = sc_.stamp;
+                sc_.markPart[nb[j]] = cellPart[i];
             }
         }
         std::vector<int> facePart, nodePart, nbPartFace(parts, 0), nbPartNode(parts, 0);
         int nbIsoFace = 0, nbIsoNode = 0;
-        items_accordingly(c2f_, t_.faceRev, fc, cellPart, ff, lf, facePart, nbPartFace, nbIsoFace);
-        if (lf - ff + 1 > 0) stable_by_part(t_.faceRev, ff, facePart);
-        items_accordingly(c2n_, t_.nodeRev, fc, cellPart, fn, ln, nodePart, nbPartNode, nbIsoNode);
-        if (ln - fn + 1 > 0) stable_by_part(t_.nodeRev, fn, nodePart);
-        stable_by_part(t_.cellRev, fc, cellPart);
+        items_accordingly(S.c2f, S.t.faceRev, fc, cellPart, ff, lf, facePart, nbPartFace, nbIsoFace);
+        if (lf - ff + 1 > 0) stable_by_part(S.t.faceRev, ff, facePart);
+        items_accordingly(S.c2n, S.t.nodeRev, fc, cellPart, fn, ln, nodePart, nbPartNode, nbIsoNode);
+        if (ln - fn + 1 > 0) stable_by_part(S.t.nodeRev, fn, nodePart);
+        stable_by_part(S.t.cellRev, fc, cellPart);
 
         const int nbIsoItem = std::max(nbIsoCell, std::max(nbIsoFace, nbIsoNode));
         std::vector<int> nbPartItem(parts);
         for (int i = 0; i < parts; ++i) nbPartItem[i] = std::max(nbPartCell[i], std::max(nbPartFace[i], nbPartNode[i]));
         init_node(tree, fc, lc, ff, lf, fn, ln, isIso);
-        t_.nodes[tree].nbParts = parts;
-        t_.nodes[tree].son.assign(parts, -1);
+        (*N)[tree].nbParts = parts;
+        (*N)[tree].son.assign(parts, -1);
         for (int i = 0; i < parts; ++i)
             if (nbPartItem[i] > 0) {
                 int s = new_node();
-                t_.nodes[tree].son[i] = s;
+                (*N)[tree].son[i] = s;
             }
         if (nbIsoItem > 0) {
             int s = new_node();
-            t_.nodes[tree].iso = s;
+            (*N)[tree].iso = s;
         }
-        bool leaf = true;
-        int st1 = fc, st2 = ff, st3 = fn;
-        for (int i = 0; i < parts; ++i) {
-            const int ed1 = st1 + nbPartCell[i] - 1;
-            const int ed2 = st2 + nbPartFace[i] - 1;
-            const int ed3 = st3 + nbPartNode[i] - 1;
-            if (nbPartItem[i] > 0) {
-                leaf = false;
-                create_normal(t_.nodes[tree].son[i], st1, ed1, st2, ed2, st3, ed3, isIso);
+        // The parts' subtrees are independent: disjoint cell, face and node
+        // ranges, and no face or node shared between two parts (a cell
+        // touching another part's node is a separator cell).  Large ones are
+        // built side by side, each into its own node list whose entry 0 is
+        // the part's node; the lists are spliced in part order afterwards,
+        // which gives every node the index the serial recursion gives it.
+        struct Job {
+            int i, st1, ed1, st2, ed2, st3, ed3;
+            std::vector<TDTreeNode> nodes;
+        };
+        std::vector<Job> jobs;
+        {
+            int st1 = fc, st2 = ff, st3 = fn;
+            for (int i = 0; i < parts; ++i) {
+                const int ed1 = st1 + nbPartCell[i] - 1;
+                const int ed2 = st2 + nbPartFace[i] - 1;
+                const int ed3 = st3 + nbPartNode[i] - 1;
+                if (nbPartItem[i] > 0) jobs.push_back({i, st1, ed1, st2, ed2, st3, ed3, {}});
+                st1 = ed1 + 1;
+                st2 = ed2 + 1;
+                st3 = ed3 + 1;
             }
-            st1 = ed1 + 1;
-            st2 = ed2 + 1;
-            st3 = ed3 + 1;
         }
-        t_.nodes[tree].isLeaf = leaf;
+        const bool leaf = jobs.empty();
+        if (nCell >= kParallelCells && jobs.size() > 1) {
+            std::vector<std::thread> th(jobs.size());
+            std::vector<std::exception_ptr> err(jobs.size());
+            std::vector<TDTreeNode>* const mine = N;
+            for (size_t j = 0; j < jobs.size(); ++j) {
+                Job& J = jobs[j];
+                J.nodes.emplace_back();
+                if (j + 1 < jobs.size() && S.take_thread()) {
+                    th[j] = std::thread([this, &J, &err, j, isIso] {
+                        try {
+                            Worker w(S, &J.nodes);
+                            w.create_normal(0, J.st1, J.ed1, J.st2, J.ed2, J.st3, J.ed3, isIso);
+                        } catch (...) {
+                            err[j] = std::current_exception();
+                        }
+                        S.spare.fetch_add(1);
+                    });
+                } else {
+                    N = &J.nodes;
+                    create_normal(0, J.st1, J.ed1, J.st2, J.ed2, J.st3, J.ed3, isIso);
+                    N = mine;
+                }
+            }
+            for (size_t j = 0; j < jobs.size(); ++j)
+                if (th[j].joinable()) th[j].join();
+            for (const std::exception_ptr& e : err)
+                if (e) std::rethrow_exception(e);
+            for (Job& J : jobs) {
+                const int g = (*N)[tree].son[J.i];
+                const int base = (int)N->size();
+                auto map = [g, base](int k) { return k < 0 ? k : (k == 0 ? g : base + k - 1); };
+                for (size_t k = 0; k < J.nodes.size(); ++k) {
+                    TDTreeNode d = std::move(J.nodes[k]);
+                    for (int& x : d.son) x = map(x);
+                    d.iso = map(d.iso);
+                    if (k == 0)
+                        (*N)[g] = std::move(d);
+                    else
+                        N->push_back(std::move(d));
+                }
+            }
+        } else {
+            for (const Job& J : jobs) create_normal((*N)[tree].son[J.i], J.st1, J.ed1, J.st2, J.ed2, J.st3, J.ed3, isIso);
+        }
+        (*N)[tree].isLeaf = leaf;
         if (nbIsoItem > 0 && !leaf)
-            create_normal(t_.nodes[tree].iso, lc - nbIsoCell + 1, lc, lf - nbIsoFace + 1, lf, ln - nbIsoNode + 1, ln, true);
+            create_normal((*N)[tree].iso, lc - nbIsoCell + 1, lc, lf - nbIsoFace + 1, lf, ln - nbIsoNode + 1, ln, true);
         else
-            t_.nodes[tree].nbParts = 0;
+            (*N)[tree].nbParts = 0;
     }
 };
 
@@ -441,8 +567,15 @@ void TDTree::leaves_forward(int node, std::vector<int>& out) const {
 }
 
 TDTree build_tdtree(const RawMesh& m, int nbParts, int partSize) {
-    Builder b(m, nbParts, partSize);
-    return b.run();
+    Shared s(m, nbParts, partSize);
+    std::vector<TDTreeNode> nodes;
+    Worker(s, &nodes).run_root();
+    TDTree& t = s.t;
+    t.nodes = std::move(nodes);
+    for (int i = 0; i < m.nC; ++i) t.cellPerm[t.cellRev[i]] = i;
+    for (int i = 0; i < m.nF; ++i) t.facePerm[t.faceRev[i]] = i;
+    for (int i = 0; i < m.nN; ++i) t.nodePerm[t.nodeRev[i]] = i;
+    return std::move(t);
 }
 
 RawMesh renumber(const RawMesh& m, const TDTree& t) {
