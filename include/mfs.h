@@ -117,6 +117,12 @@ int mfs_grid_decouple_ex(mfs_grid* g, int nb_parts, int part_size);
 /* PreprocessGrid (load_grid.cpp:405-540): SpecifyBC, ComputeMetrics,
  * AllocAndInitFlowfield (freestream), SetGhostVariables. */
 int mfs_grid_preprocess(mfs_grid* g, const mfs_params* p);
+/* The same with ComputeMetrics (grid_polyhedra_auxiliary_functions.cpp:
+ * 68-808: face centres, normals and areas, cell volumes and centroids,
+ * ghost geometry, the normal and centroid corrections) on the current HIP
+ * device; the cell-to-face lists are built there too.  The results land in
+ * the grid's arrays, bitwise equal to mfs_grid_preprocess. */
+int mfs_grid_preprocess_device(mfs_grid* g, const mfs_params* p);
 int mfs_grid_get_info(const mfs_grid* g, mfs_grid_info* info);
 /* Copies a named grid array ("x","f2c","xfn","area","vol","xcc","rho",
  * "bctype", "tdtree_leaves", ...) into dst; bytes must equal its size.

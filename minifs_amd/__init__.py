@@ -76,8 +76,11 @@ class Grid:
         check(lib().mfs_grid_decouple_ex(self._h, nb_parts, part_size), "mfs_grid_decouple_ex")
         return self
 
-    def preprocess(self, params: _lib.MfsParams) -> "Grid":
-        check(lib().mfs_grid_preprocess(self._h, C.byref(params)), "mfs_grid_preprocess")
+    def preprocess(self, params: _lib.MfsParams, device: bool = False) -> "Grid":
+        """PreprocessGrid; with device=True its ComputeMetrics runs on the GPU
+        (bitwise the same results)."""
+        fn = lib().mfs_grid_preprocess_device if device else lib().mfs_grid_preprocess
+        check(fn(self._h, C.byref(params)), "mfs_grid_preprocess")
         return self
 
     @property

@@ -78,6 +78,7 @@ _SIGS = {
     "mfs_grid_save": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "mfs_grid_decouple": (C.c_int, [C.c_void_p]),
     "mfs_grid_decouple_ex": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mfs_grid_preprocess_device": (C.c_int, [C.c_void_p, C.POINTER(MfsParams)]),
     "mfs_grid_preprocess": (C.c_int, [C.c_void_p, C.POINTER(MfsParams)]),
     "mfs_grid_get_info": (C.c_int, [C.c_void_p, C.POINTER(MfsGridInfo)]),
     "mfs_grid_array_bytes": (C.c_size_t, [C.c_void_p, C.c_char_p]),

@@ -135,6 +135,19 @@ extern "C" int mfs_grid_preprocess(mfs_grid* g, const mfs_params* p) {
     return MFS_OK;
 }
 
+extern "C" int mfs_grid_preprocess_device(mfs_grid* g, const mfs_params* p) {
+    if (!g || !p) {
+        set_error("mfs_grid_preprocess_device: null argument");
+        return MFS_ERR_ARG;
+    }
+    std::string e = g->g.preprocess(*p, true);
+    if (!e.empty()) {
+        set_error("mfs_grid_preprocess_device: " + e);
+        return MFS_ERR_MESH;
+    }
+    return MFS_OK;
+}
+
 extern "C" int mfs_grid_get_info(const mfs_grid* g, mfs_grid_info* info) {
     if (!g || !info) {
         set_error("mfs_grid_get_info: null argument");

@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <sstream>
+#include <vector>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,7 +19,10 @@
 
 #include "mfs_ctx.hpp"
 #include "mfs_mesh.hpp"
+#include "mfs_grid.hpp"
 #include "mfs_prep.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 namespace mfs {
 void set_error(const std::string& msg);  // mfs_capi.cpp
@@ -61,8 +66,144 @@ constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
 #include "mfs_kernels.hpp"
 #include "mfs_sweep.hpp"
 #include "mfs_batch.hpp"
+#include "mfs_setup.hpp"
 
 }  // namespace dev
+
+// ===========================================================================
+// ComputeMetrics on the current device (the kernels of mfs_setup.hpp); the
+// results land in the grid's host arrays, bitwise those of compute_metrics.
+namespace {
+
+struct DevScratch {
+    std::vector<void*> bufs;
+    hipStream_t stream = nullptr;
+    ~DevScratch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* b : bufs) (void)hipFree(b);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace
+
+std::string compute_metrics_device(Grid& g) {
+    const RawMesh& m = g.m;
+    const int nC = m.nC, nB = m.nB, nF = m.nF, nN = m.nN, n = nC + nB;
+    if (nC <= 0 || nF <= 0) return "empty mesh";
+    const std::vector<int> fno = m.f2n_offsets();
+    DevScratch d;
+    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) return "HIP stream creation failed";
+    hipStream_t st = d.stream;
+    dev::MetricArgs A{};
+    A.nC = nC;
+    A.nB = nB;
+    A.nF = nF;
+    int* dfno = d.alloc<int>(fno.size());
+    int* df2n = d.alloc<int>(m.f2n.size());
+    int* df2c = d.alloc<int>(m.f2c.size());
+    double* dx = d.alloc<double>(nN);
+    double* dy = d.alloc<double>(nN);
+    double* dz = d.alloc<double>(nN);
+    int* cnt = d.alloc<int>((size_t)nC + 1);
+    int* cfo = d.alloc<int>((size_t)nC + 1);
+    int* cursor = d.alloc<int>(nC);
+    int* cf = d.alloc<int>(2 * (size_t)nF - nB);
+    double* fbuf = d.alloc<double>(7 * (size_t)nF);
+    double* cbuf = d.alloc<double>(4 * (size_t)n + 3 * (size_t)nC);
+    int* small = d.alloc<int>(1);
+    if (!dfno || !df2n || !df2c || !dx || !dy || !dz || !cnt || !cfo || !cursor || !cf || !fbuf || !cbuf || !small)
+        return "device memory allocation failed";
+    bool ok = hipMemcpyAsync(dfno, fno.data(), fno.size() * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(df2n, m.f2n.data(), m.f2n.size() * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(df2c, m.f2c.data(), m.f2c.size() * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(dx, m.x.data(), nN * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(dy, m.y.data(), nN * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(dz, m.z.data(), nN * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemsetAsync(cnt, 0, ((size_t)nC + 1) * sizeof(int), st) == hipSuccess &&
+              hipMemsetAsync(cursor, 0, (size_t)nC * sizeof(int), st) == hipSuccess &&
+              hipMemsetAsync(small, 0, sizeof(int), st) == hipSuccess;
+    if (!ok) return "upload failed";
+    const unsigned gF = (unsigned)((nF + dev::kBlock - 1) / dev::kBlock);
+    const unsigned gC = (unsigned)((nC + dev::kBlock - 1) / dev::kBlock);
+    const unsigned gB = (unsigned)std::max(1, (nB + dev::kBlock - 1) / dev::kBlock);
+    // cell -> faces in ascending face id
+    hipLaunchKernelGGL(dev::k_c2f_count, dim3(gF), dim3(dev::kBlock), 0, st, nF, nB, df2c, cnt);
+    size_t tmp_bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, cfo, nC + 1, st) != hipSuccess) return "scan failed";
+    void* tmp = d.alloc<char>(tmp_bytes);
+    if (!tmp || hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, cfo, nC + 1, st) != hipSuccess)
+        return "scan failed";
+    hipLaunchKernelGGL(dev::k_c2f_fill, dim3(gF), dim3(dev::kBlock), 0, st, nF, nB, df2c, cfo, cursor, cf);
+    hipLaunchKernelGGL(dev::k_c2f_sort, dim3(gC), dim3(dev::kBlock), 0, st, nC, cfo, cf);
+    A.fno = dfno;
+    A.f2n = df2n;
+    A.f2c = df2c;
+    A.x = dx;
+    A.y = dy;
+    A.z = dz;
+    A.cfo = cfo;
+    A.cf = cf;
+    A.xfc = fbuf;
+    A.yfc = fbuf + (size_t)nF;
+    A.zfc = fbuf + 2 * (size_t)nF;
+    A.xfn = fbuf + 3 * (size_t)nF;
+    A.yfn = fbuf + 4 * (size_t)nF;
+    A.zfn = fbuf + 5 * (size_t)nF;
+    A.area = fbuf + 6 * (size_t)nF;
+    A.xcc = cbuf;
+    A.ycc = cbuf + (size_t)n;
+    A.zcc = cbuf + 2 * (size_t)n;
+    A.vol = cbuf + 3 * (size_t)n;
+    A.xcct = cbuf + 4 * (size_t)n;
+    A.ycct = A.xcct + nC;
+    A.zcct = A.ycct + nC;
+    A.small = small;
+    hipLaunchKernelGGL(dev::k_face_avg, dim3(gF), dim3(dev::kBlock), 0, st, A);
+    hipLaunchKernelGGL(dev::k_cell_avg, dim3(gC), dim3(dev::kBlock), 0, st, A);
+    hipLaunchKernelGGL(dev::k_face_geom, dim3(gF), dim3(dev::kBlock), 0, st, A);
+    hipLaunchKernelGGL(dev::k_cell_vol, dim3(gC), dim3(dev::kBlock), 0, st, A);
+    hipLaunchKernelGGL(dev::k_cell_mark, dim3(gC), dim3(dev::kBlock), 0, st, A);
+    if (nB > 0) hipLaunchKernelGGL(dev::k_ghost_geom, dim3(gB), dim3(dev::kBlock), 0, st, A);
+    hipLaunchKernelGGL(dev::k_face_normal_fix, dim3(gF), dim3(dev::kBlock), 0, st, A);
+    hipLaunchKernelGGL(dev::k_cell_centroid_fix, dim3(gC), dim3(dev::kBlock), 0, st, A, centroid_scale());
+    if (hipGetLastError() != hipSuccess) return "metric kernel launch failed";
+    int nsmall = 0;
+    g.xfc.resize(nF);
+    g.yfc.resize(nF);
+    g.zfc.resize(nF);
+    g.xfn.resize(nF);
+    g.yfn.resize(nF);
+    g.zfn.resize(nF);
+    g.area.resize(nF);
+    g.xcc.resize(n);
+    g.ycc.resize(n);
+    g.zcc.resize(n);
+    g.vol.resize(n);
+    std::vector<double>* fo[7] = {&g.xfc, &g.yfc, &g.zfc, &g.xfn, &g.yfn, &g.zfn, &g.area};
+    std::vector<double>* co[4] = {&g.xcc, &g.ycc, &g.zcc, &g.vol};
+    ok = hipMemcpyAsync(&nsmall, small, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess;
+    for (int k = 0; k < 7 && ok; ++k)
+        ok = hipMemcpyAsync(fo[k]->data(), fbuf + k * (size_t)nF, nF * sizeof(double), hipMemcpyDeviceToHost, st) ==
+             hipSuccess;
+    for (int k = 0; k < 4 && ok; ++k)
+        ok = hipMemcpyAsync(co[k]->data(), cbuf + k * (size_t)n, n * sizeof(double), hipMemcpyDeviceToHost, st) ==
+             hipSuccess;
+    if (!ok || hipStreamSynchronize(st) != hipSuccess) return "metric download failed";
+    if (nsmall > 0) {
+        std::ostringstream os;
+        os << "There are " << nsmall << " cell's vol too small! This must correct!";
+        return os.str();
+    }
+    return "";
+}
 
 // ===========================================================================
 // Context management and launch sequences.

@@ -94,6 +94,11 @@ std::string specify_bc(Grid& g) {
 
 // ComputeMetrics (grid_polyhedra.cpp:1071-1100) and its five steps
 // (grid_polyhedra_auxiliary_functions.cpp:68-808).
+double centroid_scale() {
+    const int iexp = 16;  // CorrectCellCentroid (:679-690)
+    return pow(10.0, -iexp + 1);
+}
+
 std::string compute_metrics(Grid& g) {
     const RawMesh& m = g.m;
     const int nC = m.nC, nB = m.nB, nF = m.nF, n = nC + nB;
@@ -405,8 +410,7 @@ std::string compute_metrics(Grid& g) {
 
     // CorrectCellCentroid (:679-808)
     {
-        const int iexp = 16;
-        const double scaleXYZcc = pow(10.0, -iexp + 1);
+        const double scaleXYZcc = centroid_scale();
         const double minDn = 1.0e-10;
         auto absmax_signfirst = [](double a, double b) {  // AbsMaxSignFirst (:812-823)
             const int sign = (a > 0) ? 1 : -1;
@@ -553,12 +557,12 @@ void set_ghost_variables(Grid& g) {
     }
 }
 
-std::string Grid::preprocess(const mfs_params& prm) {
+std::string Grid::preprocess(const mfs_params& prm, bool device_metrics) {
     if (preprocessed) return "grid already preprocessed";
     params = prm;
     std::string e = specify_bc(*this);
     if (!e.empty()) return e;
-    e = compute_metrics(*this);
+    e = device_metrics ? compute_metrics_device(*this) : compute_metrics(*this);
     if (!e.empty()) return e;
     init_flowfield(*this);
     set_ghost_variables(*this);

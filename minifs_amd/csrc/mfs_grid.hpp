@@ -28,12 +28,16 @@ struct Grid {
 
     // Set-up steps; return "" on success or an error message.
     std::string decouple(int nbParts = 4, int partSize = 512);
-    std::string preprocess(const mfs_params& prm);
+    std::string preprocess(const mfs_params& prm, bool device_metrics = false);
 };
 
 // Pieces of PreprocessGrid, exposed for tests.
 std::string specify_bc(Grid& g);
 std::string compute_metrics(Grid& g);
+// The scale of CorrectCellCentroid's push, 10^(1-16), shared by both paths.
+double centroid_scale();
+// The same on the current HIP device (mfs_device.hip), bitwise equal.
+std::string compute_metrics_device(Grid& g);
 void init_flowfield(Grid& g);
 void set_ghost_variables(Grid& g);
 

@@ -278,6 +278,18 @@ def test_solver_without_gpu_fails_loudly():
         mfs.Solver(g, mfs.make_params(0.5, 0.0, 10.0))
 
 
+def test_device_preprocess_without_gpu_fails_loudly():
+    from tests.conftest import gpu_available
+
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    g = mfs.Grid.box_bump(2).decouple()
+    with pytest.raises(mfs.MfsError):
+        g.preprocess(mfs.make_params(0.5, 0.0, 10.0), device=True)
+    assert g.info["preprocessed"] == 0
+    g.close()
+
+
 def test_native_driver_without_gpu_fails_loudly(tmp_path):
     """The native driver (mfs_solve) builds the grid on the host, then stops
     with an error at the device context when there is no GPU; it also
