@@ -4,8 +4,9 @@ Requests arrive over time.  Each one is a box+bump mesh (size, bump height,
 seed, cell kind) and a flow condition, solved to a residual drop.  The
 server overlaps three things:
 
-* host set-up (mesh generation, TDTree ordering, metrics: the C++ host code
-  releases the GIL through ctypes) on a pool of worker threads;
+* set-up (mesh generation and TDTree ordering on the host, ComputeMetrics on
+  the GPU; the C++ code releases the GIL through ctypes) on a pool of worker
+  threads;
 * admission of set-up solves while the device memory they need fits the
   budget: their device contexts (host layout, upload) are built on worker
   threads and then attached to the one device batch (mfs_batch_attach);
@@ -75,10 +76,10 @@ def make_requests(n: int, rate: float, cells_lo: float, cells_hi: float, seed: i
     return out
 
 
-def _setup(r: Request, cfl: float, tree: tuple[int, int] = (4, 512)):
+def _setup(r: Request, cfl: float, tree: tuple[int, int] = (4, 512), device_metrics: bool = True):
     p = make_params(mach=r.mach, alpha=r.alpha, cfl=cfl)
     g = Grid.box_bump(r.nx, r.ny, r.nz, bump_height=r.bump_height, seed=r.seed, cells=r.cells)
-    g.decouple(*tree).preprocess(p)
+    g.decouple(*tree).preprocess(p, device=device_metrics)
     return g, p
 
 
@@ -89,7 +90,7 @@ def _converged(h: np.ndarray, drop: float) -> bool:
 class Server:
     def __init__(self, precision: int, hbm_budget: float, drop: float = 1e-8, max_iters: int = 20000,
                  check_every: int = 20, cfl: float = 10.0, workers: int = 8, bytes_per_cell: float | None = None,
-                 tree: tuple[int, int] = (4, 512)):
+                 tree: tuple[int, int] = (4, 512), device_metrics: bool = True):
         self.precision = precision
         self.budget = hbm_budget
         self.drop = drop
@@ -98,6 +99,7 @@ class Server:
         self.cfl = cfl
         self.workers = workers
         self.tree = tuple(tree)     # TDTree parameters of the meshes' cell order
+        self.device_metrics = device_metrics  # ComputeMetrics on the GPU (bitwise the host's)
         # device bytes per cell, refined from the contexts actually created
         self.bytes_per_cell = bytes_per_cell or (620.0 if precision == MFS_FP64 else 900.0)
 
@@ -121,7 +123,7 @@ class Server:
         while len(done) < len(reqs):
             t = now()
             while nxt < len(reqs) and reqs[nxt].arrival <= t:
-                building[setup_pool.submit(_setup, reqs[nxt], self.cfl, self.tree)] = reqs[nxt]
+                building[setup_pool.submit(_setup, reqs[nxt], self.cfl, self.tree, self.device_metrics)] = reqs[nxt]
                 nxt += 1
             for fut in [f for f in building if f.done()]:
                 r = building.pop(fut)
@@ -184,7 +186,7 @@ class Server:
 
 def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int = 20000, check_every: int = 20,
                    cfl: float = 10.0, precision: int = MFS_FP64, progress=None,
-                   tree: tuple[int, int] = (4, 512)) -> dict:
+                   tree: tuple[int, int] = (4, 512), device_metrics: bool = True) -> dict:
     """The baseline: requests served one at a time in arrival order, each set
     up and solved alone."""
     reqs = sorted(requests, key=lambda r: r.arrival)
@@ -195,7 +197,7 @@ def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int =
         wait = r.arrival - now()
         if wait > 0:
             time.sleep(wait)
-        g, p = _setup(r, cfl, tree)
+        g, p = _setup(r, cfl, tree, device_metrics)
         r.t_ready = r.t_admit = now()
         s = Solver(g, p, precision=precision)
         r.n_cells = g.info["nTCell"]
