@@ -704,8 +704,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     ok = ok && dalloc(ctx, ctx->crun, (size_t)ctx->nchunk, "chunk records") &&
          dalloc(ctx, ctx->pos, (size_t)ctx->nchunk, "chunk positions") &&
          dalloc(ctx, ctx->tile_chunk, L.tile_chunk.size(), "tile_chunk") &&
-         dalloc(ctx, ctx->snb, L.snb.size(), "slot neighbours") &&
-         dalloc(ctx, ctx->sgeo, L.snb.size(), "slot geometry") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
+         dalloc(ctx, ctx->snb, L.cfs.size(), "slot neighbours") &&
+         dalloc(ctx, ctx->sgeo, L.cfs.size(), "slot geometry") && dalloc(ctx, ctx->depmask, L.nC, "depmask");
     const bool mixed = precision == MFS_MIXED;
     if (mixed) {
         ok = ok && dalloc(ctx, ctx->q32, 5 * n, "q32") && dalloc(ctx, ctx->diag32, L.nC, "diag32") &&
@@ -713,14 +713,14 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              dalloc(ctx, ctx->gx32, 5 * n, "dqdx32") && dalloc(ctx, ctx->gy32, 5 * n, "dqdy32") &&
              dalloc(ctx, ctx->gz32, 5 * n, "dqdz32") && dalloc(ctx, ctx->qn32, 5 * (size_t)L.nN, "qnode32") &&
              dalloc(ctx, ctx->ell_w32, ell_w.size(), "ell_w32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32") &&
-             dalloc(ctx, ctx->sgeo32, L.snb.size(), "slot geometry (FP32)");
+             dalloc(ctx, ctx->sgeo32, L.cfs.size(), "slot geometry (FP32)");
     }
     ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err") &&
          dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets") &&
          dalloc(ctx, ctx->gface, (size_t)L.nB, "ghost faces");
-    if (!L.snode.empty()) ok = ok && dalloc(ctx, ctx->snode, L.snode.size(), "slot nodes");
-    ok = ok && dalloc(ctx, ctx->sdn, L.sdn.size(), "slot distances") && dalloc(ctx, ctx->sflip, L.sflip.size(), "slot sides");
+    if (L.tri_slots) ok = ok && dalloc(ctx, ctx->snode, 12 * (size_t)L.nC, "slot nodes");
+    ok = ok && dalloc(ctx, ctx->sdn, L.cfs.size(), "slot distances") && dalloc(ctx, ctx->sflip, L.nC, "slot sides");
     ok = ok && ensure_hist(ctx, 64);
     if (ok) {
         // resident workgroups of the sweep launch
@@ -765,17 +765,28 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              upload(ctx, ctx->crun, reinterpret_cast<const int4*>(L.crun.data()), (size_t)ctx->nchunk, "upload crun") &&
              upload(ctx, ctx->pos, L.pos.data(), L.pos.size(), "upload pos") &&
              upload(ctx, ctx->tile_chunk, L.tile_chunk.data(), L.tile_chunk.size(), "upload tile_chunk") &&
-             upload(ctx, ctx->snb, L.snb.data(), L.snb.size(), "upload snb") &&
-             upload(ctx, ctx->sgeo, reinterpret_cast<const double4*>(L.sgeo.data()), L.snb.size(), "upload sgeo") &&
              upload(ctx, ctx->depmask, L.depmask.data(), L.nC, "upload depmask") &&
-             upload(ctx, ctx->gface, gf.data(), (size_t)L.nB, "upload gface") &&
-             (L.snode.empty() || upload(ctx, ctx->snode, L.snode.data(), L.snode.size(), "upload snode")) &&
-             upload(ctx, ctx->sdn, L.sdn.data(), L.sdn.size(), "upload sdn") &&
-             upload(ctx, ctx->sflip, L.sflip.data(), L.sflip.size(), "upload sflip");
+             upload(ctx, ctx->gface, gf.data(), (size_t)L.nB, "upload gface");
+        // the slot tables, built on the device from the slot face ids
+        int* dcfs = nullptr;
+        ok = ok && hip_ok(ctx, hipMalloc((void**)&dcfs, L.cfs.size() * sizeof(int)), "hipMalloc(slot faces)");
+        if (ok) {
+            ok = upload(ctx, dcfs, L.cfs.data(), L.cfs.size(), "upload slot faces");
+            if (ok) {
+                const unsigned gC = (unsigned)((L.nC + kBlock - 1) / kBlock);
+                auto kern = L.lpc == 4 ? dev::k_slot_tables<4> : dev::k_slot_tables<8>;
+                hipLaunchKernelGGL(kern, dim3(gC), dim3(kBlock), 0, ctx->stream, L.nC, dcfs, ctx->cfo, ctx->fcell,
+                                   ctx->fgeo, ctx->fdn, ctx->fno, ctx->fn, ctx->snb, ctx->sgeo,
+                                   mixed ? ctx->sgeo32 : nullptr, ctx->sdn, ctx->sflip, ctx->snode);
+                ok = check_launch(ctx, "k_slot_tables") &&
+                     hip_ok(ctx, hipStreamSynchronize(ctx->stream), "slot tables");
+            }
+            (void)hipFree(dcfs);
+        }
         const double run0[8] = {0.0, kBIGd, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         ok = ok && upload(ctx, ctx->run, run0, 8, "upload run");
         std::vector<float> f32;
-        std::vector<float4> g32, s32;
+        std::vector<float4> g32;
         if (ok && mixed) {
             // FP32 copies of the operator's static inputs; the ghost part of
             // the state snapshot is fixed (ghosts are frozen after set-up)
@@ -785,12 +796,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
                                      (float)L.fgeo[4 * (size_t)i + 2], (float)L.fgeo[4 * (size_t)i + 3]);
             f32.resize(std::max(ell_w.size(), 5 * n));
             for (size_t k = 0; k < ell_w.size(); ++k) f32[k] = (float)ell_w[k];
-            s32.resize(L.snb.size());
-            for (size_t i = 0; i < L.snb.size(); ++i)
-                s32[i] = make_float4((float)L.sgeo[4 * i], (float)L.sgeo[4 * i + 1], (float)L.sgeo[4 * i + 2],
-                                     (float)L.sgeo[4 * i + 3]);
             ok = upload(ctx, ctx->fgeo32, g32.data(), L.nF, "upload fgeo32") &&
-                 upload(ctx, ctx->sgeo32, s32.data(), s32.size(), "upload sgeo32") &&
                  upload(ctx, ctx->ell_w32, f32.data(), ell_w.size(), "upload ell_w32");
             for (size_t k = 0; k < 5 * n; ++k) f32[k] = (float)qa[k];
             ok = ok && upload(ctx, ctx->q32, f32.data(), 5 * n, "upload q32");

@@ -463,42 +463,18 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
             L.crun[4 * (size_t)u + 2] = L.rep_f[t];
             L.crun[4 * (size_t)u + 3] = L.rep_b[t];
         }
-        // tetrahedral meshes: each slot's three face nodes in the face's own
-        // order (the gradient's node mean adds them in that order)
-        L.snode.clear();
+        // tetrahedral meshes whose slots are all triangles take the
+        // gradient's slot-node path (the device builds the slot tables from
+        // cfs and the face arrays)
+        L.tri_slots = false;
         if (L.max_faces == 4) {
             bool tri = true;
-            L.snode.assign(12 * (size_t)nC, 0);
-            for (int d = 0; d < nC && tri; ++d)
-                for (int sl = 0; sl < 4 && tri; ++sl) {
-                    const int f = L.cfs[(size_t)d * 4 + sl];
-                    tri = f >= 0 && L.fno[f + 1] - L.fno[f] == 3;
-                    if (tri)
-                        for (int i = 0; i < 3; ++i) L.snode[12 * (size_t)d + 3 * sl + i] = L.fn[L.fno[f] + i];
-                }
-            if (!tri) L.snode.clear();  // not all triangles: the gradient keeps its face-list path
-        }
-        L.snb.resize((size_t)nC * L.lpc);
-        L.sgeo.assign(4 * (size_t)nC * L.lpc, 0.0);
-        L.sdn.assign((size_t)nC * L.lpc, 0.0);
-        L.sflip.assign(nC, 0);
-        for (int d = 0; d < nC; ++d)
-            L.sflip[d] = (L.cfo[d + 1] - L.cfo[d]) << 8;
-        for (int d = 0; d < nC; ++d)
-            for (int sl = 0; sl < L.lpc; ++sl) {
-                const size_t i = (size_t)d * L.lpc + sl;
+            for (size_t i = 0; i < L.cfs.size() && tri; ++i) {
                 const int f = L.cfs[i];
-                if (f < 0) {
-                    L.snb[i] = d;
-                    continue;
-                }
-                const bool flip = (L.c2[f] == d);
-                if (flip) L.sflip[d] |= 1 << sl;
-                L.sdn[i] = flip ? L.dn2[f] : L.dn1[f];
-                L.snb[i] = flip ? L.c1[f] : L.c2[f];
-                for (int j = 0; j < 3; ++j) L.sgeo[4 * i + j] = flip ? -L.fgeo[4 * (size_t)f + j] : L.fgeo[4 * (size_t)f + j];
-                L.sgeo[4 * i + 3] = L.fgeo[4 * (size_t)f + 3];
+                tri = f >= 0 && L.fno[f + 1] - L.fno[f] == 3;
             }
+            L.tri_slots = tri;
+        }
     }
 
     // ---- state, device order (ghosts unchanged)

@@ -69,20 +69,13 @@ struct HostLayout {
     std::vector<int> cfs;                      // lpc slots per device cell: device face id or -1
     // what the sweeps read, laid out so that a chunk needs no chain of
     // lookups: per position u of `run`, {first device cell, cell count
-    // (| kThinChunkFlag on a thin level), rep_f, rep_b} of chunk run[u]; per chunk, its position; per device
-    // cell and slot, the neighbour across the slot's face (the cell itself
-    // when the slot is empty) and the face's unit normal pointing away from
-    // the cell with its area (zeros when empty)
+    // (| kThinChunkFlag on a thin level), rep_f, rep_b} of chunk run[u]; per
+    // chunk, its position.  The per-slot tables (neighbour, oriented normal
+    // and area, side distance, sides, tetrahedra's face nodes) are built on
+    // the device from cfs and the face arrays (k_slot_tables).
     std::vector<int> crun;                     // 4 per position
     std::vector<int> pos;                      // per chunk
-    std::vector<int> snb;                      // lpc per device cell
-    std::vector<int> snode;                    // tetrahedral meshes: 3 face nodes per slot, 12 per cell
-    // per device cell: bits 0-7 slots whose face has this cell as its second
-    // cell (c2), bits 8-11 the number of faces; per slot the distance
-    // |(x_fc - x_cc).n| on this cell's side (0 when empty)
-    std::vector<int> sflip;
-    std::vector<double> sdn;
-    std::vector<double> sgeo;                  // 4*lpc per device cell
+    bool tri_slots = false;                    // tetrahedra with triangular slots only
     // initial state, 5 blocks of nC+nB, device cell order
     std::vector<double> q;
 };

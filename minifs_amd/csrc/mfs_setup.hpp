@@ -324,3 +324,47 @@ __global__ __launch_bounds__(kBlock) void k_cell_centroid_fix(MetricArgs A, doub
     A.ycc[c] = yc;
     A.zcc[c] = zc;
 }
+
+// The per-slot tables of the residual kernel, the sweeps and the gradient,
+// from each cell's slot face ids (its faces in ascending mesh face id):
+// the neighbour across the face (the cell itself for an empty slot), the
+// face's unit normal pointing away from the cell with its area (negated
+// exactly where the cell is the face's second cell; zeros for an empty
+// slot), the distance on this cell's side, per cell the slots where it is
+// the second cell (bits 0-7) and its face count (bits 8-11), and for
+// tetrahedra each slot's three face nodes in the face's order.
+template <int LPC>
+__global__ __launch_bounds__(kBlock) void k_slot_tables(int nC, const int* __restrict__ cfs,
+                                                       const int* __restrict__ cfo, const int2* __restrict__ fcell,
+                                                       const double4* __restrict__ fgeo,
+                                                       const double2* __restrict__ fdn, const int* __restrict__ fno,
+                                                       const int* __restrict__ fn, int* snb, double4* sgeo,
+                                                       float4* sgeo32, double* sdn, int* sflip, int* snode) {
+    const int d = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (d >= nC) return;
+    int flips = (cfo[d + 1] - cfo[d]) << 8;
+    for (int sl = 0; sl < LPC; ++sl) {
+        const size_t i = (size_t)d * LPC + sl;
+        const int f = cfs[i];
+        if (f < 0) {
+            snb[i] = d;
+            sgeo[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (sgeo32) sgeo32[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            sdn[i] = 0.0;
+            continue;
+        }
+        const int2 cc = fcell[f];
+        const bool flip = cc.y == d;
+        if (flip) flips |= 1 << sl;
+        const double4 g = fgeo[f];
+        const double4 o = flip ? make_double4(-g.x, -g.y, -g.z, g.w) : g;
+        snb[i] = flip ? cc.x : cc.y;
+        sgeo[i] = o;
+        if (sgeo32) sgeo32[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+        const double2 dn = fdn[f];
+        sdn[i] = flip ? dn.y : dn.x;
+        if (snode)
+            for (int k = 0; k < 3; ++k) snode[12 * (size_t)d + 3 * sl + k] = fn[fno[f] + k];
+    }
+    sflip[d] = flips;
+}
