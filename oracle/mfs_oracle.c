@@ -50,6 +50,8 @@ struct or_grid {
     int *cno, *cn;       /* C2N: unique nodes per cell */
     int *Nmark;
     double *WeightNode, *wbf, *wcn; /* WeightNodeBFace2C (per f2n entry), WeightNodeC2N (per cn entry) */
+    int nsymm, *symm_node;          /* FindNodeSYMM: nodes on SYMM faces */
+    double* symm_n;                 /* their mean face normal, 3 per node */
     /* level schedule for the threaded mode */
     int *flev_off, *flev_cells, nflev;
     int iter_done;
@@ -524,6 +526,36 @@ static void connectivity_and_weights(or_grid* g) {
             g->wcn[e] = wr;
             g->WeightNode[p1] += wr;
         }
+    /* FindNodeSYMM (grid_polyhedra_auxiliary_functions.cpp:2060-2139):
+     * nodes of SYMM faces with the mean of those faces' normals */
+    {
+        double* sn = calloc(4 * (size_t)nN, 8); /* x, y, z sums and face count */
+        int* mark = calloc(nN, sizeof(int));
+        for (int i = 0; i < nB; i++) {
+            if (g->btype[i] != SYMM) continue;
+            for (int j = g->fo[i]; j < g->fo[i + 1]; j++) {
+                int p1 = g->f2n[j];
+                mark[p1] = 1;
+                sn[4 * (size_t)p1] += g->xfn[i];
+                sn[4 * (size_t)p1 + 1] += g->yfn[i];
+                sn[4 * (size_t)p1 + 2] += g->zfn[i];
+                sn[4 * (size_t)p1 + 3] += 1.0;
+            }
+        }
+        g->nsymm = 0;
+        for (int i = 0; i < nN; ++i) g->nsymm += mark[i];
+        g->symm_node = malloc(sizeof(int) * (g->nsymm + 1));
+        g->symm_n = malloc(8 * (3 * (size_t)g->nsymm + 1));
+        int k = 0;
+        for (int i = 0; i < nN; ++i) {
+            if (!mark[i]) continue;
+            g->symm_node[k] = i;
+            for (int a = 0; a < 3; ++a) g->symm_n[3 * k + a] = sn[4 * (size_t)i + a] / sn[4 * (size_t)i + 3];
+            ++k;
+        }
+        free(mark);
+        free(sn);
+    }
     /* forward level schedule for the threaded mode */
     int* lev = calloc(nC, sizeof(int));
     int nl = 0;
@@ -719,6 +751,37 @@ static void sweep_cell(or_grid* g, const double* Diag, int cell, int fwd) {
     limiter(g, cell);
 }
 
+/* CompNodeVar3D_dist (cal_gradient.cpp:1922-2327) up to the division: the
+ * distance-weighted node values of one variable from the boundary faces
+ * (WALL, FAR_FIELD, other types, :2155-2240) and the cells (:2260-2280). */
+static void node_values(const or_grid* g, const double* qv, double* qn) {
+    const int nN = g->nN, nB = g->nB, nC = g->nC;
+    memset(qn, 0, 8 * (size_t)nN);
+    for (int pass = 0; pass < 3; ++pass)
+        for (int i = 0; i < nB; i++) {
+            int t = g->btype[i];
+            int take = pass == 0 ? t == WALL
+                     : pass == 1 ? t == FAR_FIELD
+                                 : !(t == WALL || t == SYMM || t == FAR_FIELD || t == INTERFACE);
+            if (!take) continue;
+            double fq = qv[g->f2c[2 * i]] + qv[g->f2c[2 * i + 1]];
+            fq *= 0.5;
+            for (int j = g->fo[i]; j < g->fo[i + 1]; j++) {
+                int p1 = g->f2n[j];
+                if (pass == 1 && g->Nmark[p1] == WALL) continue;
+                if (pass == 2 && (g->Nmark[p1] == WALL || g->Nmark[p1] == FAR_FIELD)) continue;
+                qn[p1] += fq * g->wbf[j];
+            }
+        }
+    for (int c = 0; c < nC; ++c)
+        for (int e = g->cno[c]; e < g->cno[c + 1]; ++e) {
+            int p1 = g->cn[e];
+            if (g->Nmark[p1] != 0) continue;
+            qn[p1] += qv[c] * g->wcn[e];
+        }
+    for (int i = 0; i < nN; ++i) qn[i] /= (g->WeightNode[i] + TINY);
+}
+
 static void run_iteration(or_grid* g, int threads) {
     const Par* P = &g->P;
     const int nC = g->nC, nB = g->nB, nF = g->nF, nN = g->nN;
@@ -877,32 +940,28 @@ static void run_iteration(or_grid* g, int threads) {
     memset(g->dqdx, 0, 8 * 5 * n);
     memset(g->dqdy, 0, 8 * 5 * n);
     memset(g->dqdz, 0, 8 * 5 * n);
+    double* ru = g->nsymm ? malloc(24 * (size_t)nN) : NULL; /* uncorrected node u, v, w */
+    double *rv = ru ? ru + nN : NULL, *rw = ru ? ru + 2 * (size_t)nN : NULL;
     for (int var = 0; var < 5; ++var) {
         const double* qv = q[var];
-        memset(qn, 0, 8 * (size_t)nN);
-        for (int pass = 0; pass < 3; ++pass) /* boundary faces: WALL, FAR_FIELD, other types (:2142-2240) */
-            for (int i = 0; i < nB; i++) {
-                int t = g->btype[i];
-                int take = pass == 0 ? t == WALL
-                         : pass == 1 ? t == FAR_FIELD
-                                     : !(t == WALL || t == SYMM || t == FAR_FIELD || t == INTERFACE);
-                if (!take) continue;
-                double fq = qv[g->f2c[2 * i]] + qv[g->f2c[2 * i + 1]];
-                fq *= 0.5;
-                for (int j = g->fo[i]; j < g->fo[i + 1]; j++) {
-                    int p1 = g->f2n[j];
-                    if (pass == 1 && g->Nmark[p1] == WALL) continue;
-                    if (pass == 2 && (g->Nmark[p1] == WALL || g->Nmark[p1] == FAR_FIELD)) continue;
-                    qn[p1] += fq * g->wbf[j];
-                }
+        node_values(g, qv, qn);
+        if (g->nsymm && var >= 1 && var <= 3) {
+            /* symmetry-plane nodes (:2332-2348): the node velocity loses its
+             * component along the mean SYMM normal.  vn uses the uncorrected
+             * node u, v, w, all three formed in the u call (:1989-2325). */
+            if (var == 1) {
+                memcpy(ru, qn, 8 * (size_t)nN);
+                node_values(g, q[2], rv);
+                node_values(g, q[3], rw);
             }
-        for (int c = 0; c < nC; ++c) /* cell -> node scatter (:2260-2280) */
-            for (int e = g->cno[c]; e < g->cno[c + 1]; ++e) {
-                int p1 = g->cn[e];
-                if (g->Nmark[p1] != 0) continue;
-                qn[p1] += qv[c] * g->wcn[e];
+            const double* raw = var == 1 ? ru : var == 2 ? rv : rw;
+            for (int k = 0; k < g->nsymm; ++k) {
+                const int i = g->symm_node[k];
+                const double* nrm = g->symm_n + 3 * k;
+                double vn = ru[i] * nrm[0] + rv[i] * nrm[1] + rw[i] * nrm[2];
+                qn[i] = raw[i] - vn * nrm[var - 1];
             }
-        for (int i = 0; i < nN; ++i) qn[i] /= (g->WeightNode[i] + TINY);
+        }
         double *gx = g->dqdx + var * n, *gy = g->dqdy + var * n, *gz = g->dqdz + var * n;
         for (int i = 0; i < nF; i++) { /* face loop (:1740-1795) */
             int c1 = g->f2c[2 * i], c2 = g->f2c[2 * i + 1];
@@ -932,6 +991,7 @@ static void run_iteration(or_grid* g, int threads) {
         }
     }
     free(qn);
+    free(ru);
     g->iter_done++;
 }
 
@@ -1048,7 +1108,7 @@ void or_free(or_grid* g) {
     if (!g) return;
     void* ptrs[] = {g->x, g->y, g->z, g->nNPF, g->fo, g->f2n, g->f2c, g->btype, g->xfn, g->yfn, g->zfn, g->area,
                     g->xfc, g->yfc, g->zfc, g->xcc, g->ycc, g->zcc, g->vol, g->res, g->dt, g->dq, g->dqdx, g->dqdy,
-                    g->dqdz, g->cfo, g->cf, g->cno, g->cn, g->Nmark, g->WeightNode, g->wbf, g->wcn, g->flev_off,
+                    g->dqdz, g->cfo, g->cf, g->cno, g->cn, g->Nmark, g->WeightNode, g->wbf, g->wcn, g->flev_off, g->symm_node, g->symm_n,
                     g->flev_cells};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
     for (int j = 0; j < 5; ++j) free(g->q[j]);

@@ -77,6 +77,34 @@ def test_other_tree_orders_bitwise_vs_reference(tmp_path, ref_driver, tree):
     s.close()
 
 
+@pytest.mark.parametrize("bc_text", ["2\n1 3\n2 4\n", "2\n1 3\n2 61\n", "2\n1 4\n2 6\n"],
+                         ids=["symmetry", "far-field-mod", "symmetric-bump"])
+def test_other_boundary_types_bitwise_vs_reference(tmp_path, ref_driver, bc_text):
+    """The box's patches remapped in bc.para to SYMM (4) and FAR_FIELD_MOD
+    (61): the symmetry-node values, the other-type node weights and the
+    ghost rules those types take, against the reference build."""
+    print(f"[parity] bc {bc_text!r}", flush=True)
+    mach, alpha, cfl, iters = 0.6, 2.0, 10.0, 20
+    g0 = mfs.Grid.box_bump(8, seed=5).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g0.save(mesh, bc)
+    g0.close()
+    open(bc, "w").write(bc_text)
+    out = str(tmp_path / "ref")
+    ref = run_reference(ref_driver, mesh, bc, out, mach, alpha, cfl, iters, dumps=(1, iters))
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
+    g = mfs.Grid.load(mesh, bc).preprocess(p)
+    s = mfs.Solver(g, p)
+    s.step(1)
+    _assert_state_equal(s, out, 1, g.info["nTCell"])
+    s.step(iters - 1)
+    _assert_state_equal(s, out, iters, g.info["nTCell"])
+    h = s.history()
+    np.testing.assert_array_equal(h[:, 5:], ref["history"][:, 5:])
+    np.testing.assert_allclose(h[:, :5], ref["history"][:, :5], rtol=HIST_RTOL, atol=0)
+    s.close()
+
+
 @pytest.mark.parametrize("cells", ["hex", "prism"])
 def test_polyhedral_cells_bitwise_vs_reference(tmp_path, ref_driver, cells):
     """Hexahedra (6 quadrilateral faces) and prisms (5 faces, mixed

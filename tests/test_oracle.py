@@ -104,6 +104,25 @@ def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver, tree
         np.testing.assert_array_equal(o.get(nm), load_state(str(tmp_path / "r"), 6, nm), err_msg=nm)
 
 
+@pytest.mark.parametrize("bc_text", ["2\n1 3\n2 4\n", "2\n1 3\n2 61\n"], ids=["symmetry", "far-field-mod"])
+def test_oracle_matches_reference_build_on_other_boundary_types(tmp_path, ref_driver, bc_text):
+    """SYMM and FAR_FIELD_MOD patches (bc.para remapped): symmetry nodes,
+    other-type node weights and their ghost rules."""
+    import minifs_amd as mfs
+    from tests.refrun import load_state, run_reference
+
+    g = mfs.Grid.box_bump(5, 4, 3, seed=13).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    open(bc, "w").write(bc_text)
+    ref = run_reference(ref_driver, mesh, bc, str(tmp_path / "r"), 0.6, 2.0, 10.0, 6, dumps=(6,))
+    o = Oracle(mesh, bc, 0.6, 2.0, 10.0)
+    h, _ = o.run(6, 2)
+    np.testing.assert_array_equal(h, ref["history"])
+    for nm in FIELDS:
+        np.testing.assert_array_equal(o.get(nm), load_state(str(tmp_path / "r"), 6, nm), err_msg=nm)
+
+
 def test_oracle_ghost_refresh_matches_reference_build(tmp_path, ref_driver):
     """SetGhostVariables after every update (the call main.cpp:283 leaves
     commented out): the oracle against the reference build running its own
