@@ -108,12 +108,25 @@ static void dump_state(PolyGrid* g, const std::string& dir, int it) {
 
 int main(int argc, char** argv) {
     if (argc < 9) {
-        fprintf(stderr, "usage: %s MESH BC OUTDIR MACH ALPHA CFL ITERS DUMPLIST [ghosts]\n", argv[0]);
+        fprintf(stderr, "usage: %s MESH BC OUTDIR MACH ALPHA CFL ITERS DUMPLIST [ghosts] [ramp=CFL_START,NSTEP]\n", argv[0]);
         return 2;
     }
     // "ghosts": call SetGhostVariables after PassInterfaceData in every
-    // iteration, where src/main.cpp:283 has the call commented out
-    const bool ghost_refresh = argc >= 10 && std::string(argv[9]) == "ghosts";
+    // iteration, where src/main.cpp:283 has the call commented out.
+    // "ramp=a,n": CFL ramp from cfl_start = a to cfl_end = CFL over n
+    // iterations (compute_timestep.cpp:868-871), cfl_min = 0.5 cfl_start as
+    // in para_field_global.cpp:58.
+    bool ghost_refresh = false;
+    double ramp_start = 0.0;
+    int ramp_nstep = 0;
+    for (int a = 9; a < argc; ++a) {
+        const std::string s = argv[a];
+        if (s == "ghosts") ghost_refresh = true;
+        if (s.rfind("ramp=", 0) == 0) {
+            ramp_start = atof(s.substr(5).c_str());
+            ramp_nstep = atoi(s.substr(s.find(',') + 1).c_str());
+        }
+    }
     const std::string mesh = argv[1], bcf = argv[2], out = argv[3];
     const double m_in = atof(argv[4]), a_in = atof(argv[5]), cfl = atof(argv[6]);
     const int iters = atoi(argv[7]);
@@ -129,6 +142,12 @@ int main(int argc, char** argv) {
         }
     }
     set_flow(m_in, a_in, cfl);
+    if (ramp_nstep > 0) {
+        cfl_start = ramp_start;
+        cfl_nstep = ramp_nstep;
+        cfl_min = 0.5 * cfl_start;
+        cfl_ratio = cfl_end / cfl_start;
+    }
     n_steps = iters;
 
     PolyGrid* grid = new PolyGrid();

@@ -77,12 +77,13 @@ def test_other_tree_orders_bitwise_vs_reference(tmp_path, ref_driver, tree):
     s.close()
 
 
-@pytest.mark.parametrize("bc_text", ["2\n1 3\n2 4\n", "2\n1 3\n2 61\n", "2\n1 4\n2 6\n"],
-                         ids=["symmetry", "far-field-mod", "symmetric-bump"])
+@pytest.mark.parametrize("bc_text", ["2\n1 3\n2 4\n", "2\n1 3\n2 61\n", "2\n1 4\n2 6\n", "2\n1 4\n2 10\n"],
+                         ids=["symmetry", "far-field-mod", "symmetric-bump", "interface"])
 def test_other_boundary_types_bitwise_vs_reference(tmp_path, ref_driver, bc_text):
-    """The box's patches remapped in bc.para to SYMM (4) and FAR_FIELD_MOD
-    (61): the symmetry-node values, the other-type node weights and the
-    ghost rules those types take, against the reference build."""
+    """The box's patches remapped in bc.para to SYMM (4), FAR_FIELD_MOD (61)
+    and INTERFACE (10): the symmetry-node values, the other-type node
+    weights, the node-mean face values and the ghost rules those types
+    take, against the reference build."""
     print(f"[parity] bc {bc_text!r}", flush=True)
     mach, alpha, cfl, iters = 0.6, 2.0, 10.0, 20
     g0 = mfs.Grid.box_bump(8, seed=5).decouple()
@@ -98,6 +99,30 @@ def test_other_boundary_types_bitwise_vs_reference(tmp_path, ref_driver, bc_text
     s.step(1)
     _assert_state_equal(s, out, 1, g.info["nTCell"])
     s.step(iters - 1)
+    _assert_state_equal(s, out, iters, g.info["nTCell"])
+    h = s.history()
+    np.testing.assert_array_equal(h[:, 5:], ref["history"][:, 5:])
+    np.testing.assert_allclose(h[:, :5], ref["history"][:, :5], rtol=HIST_RTOL, atol=0)
+    s.close()
+
+
+def test_cfl_ramp_bitwise_vs_reference(tmp_path, ref_driver):
+    """The CFL ramp of LimitTimeStep (compute_timestep.cpp:868-871): cfl
+    from cfl_start to cfl_end over cfl_nstep iterations, then cfl_end; the
+    run crosses the end of the ramp."""
+    print("[parity] cfl ramp 1 -> 10 over 12 iterations", flush=True)
+    mach, alpha, iters = 0.7, 1.0, 18
+    g = mfs.Grid.box_bump(7, seed=21).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    out = str(tmp_path / "ref")
+    ref = run_reference(ref_driver, mesh, bc, out, mach, alpha, 10.0, iters, dumps=(5, iters), ramp=(1.0, 12))
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl_start=1.0, cfl_end=10.0, cfl_nstep=12)
+    g.preprocess(p)
+    s = mfs.Solver(g, p)
+    s.step(5)
+    _assert_state_equal(s, out, 5, g.info["nTCell"])
+    s.step(iters - 5)
     _assert_state_equal(s, out, iters, g.info["nTCell"])
     h = s.history()
     np.testing.assert_array_equal(h[:, 5:], ref["history"][:, 5:])

@@ -104,7 +104,8 @@ def test_oracle_matches_reference_build_on_fresh_mesh(tmp_path, ref_driver, tree
         np.testing.assert_array_equal(o.get(nm), load_state(str(tmp_path / "r"), 6, nm), err_msg=nm)
 
 
-@pytest.mark.parametrize("bc_text", ["2\n1 3\n2 4\n", "2\n1 3\n2 61\n"], ids=["symmetry", "far-field-mod"])
+@pytest.mark.parametrize("bc_text", ["2\n1 3\n2 4\n", "2\n1 3\n2 61\n", "2\n1 3\n2 10\n"],
+                         ids=["symmetry", "far-field-mod", "interface"])
 def test_oracle_matches_reference_build_on_other_boundary_types(tmp_path, ref_driver, bc_text):
     """SYMM and FAR_FIELD_MOD patches (bc.para remapped): symmetry nodes,
     other-type node weights and their ghost rules."""
