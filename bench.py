@@ -55,6 +55,11 @@ KERNEL_BYTES = {
     "fwd_sweep": (40, 144, 40, 0),
     # backward sweep + update: as the forward sweep plus the state update
     "bwd_sweep": (40, 264, 40, 0),
+    # gradient of one iteration + residual/time-step pass of the next (mfs_step,
+    # tetrahedra): the union of the two: face tables (flux_ts's 56 + the 3
+    # face nodes), q + vol read once, res, dt, Diag and 15 gradient outputs
+    # written, ghosts, node values
+    "grad_flux_ts": (68, 184, 80, 40),
     # node values: 4 weighted entries per tetrahedron (index + weight), q, node write
     "node": (0, 88, 76, 52),
     # Green-Gauss: face nodes + geometry, q, node values, vol, 15 gradient outputs

@@ -462,8 +462,11 @@ bool launch_sweeps(mfs_ctx* ctx) {
     return ctx->lpc == 4 ? launch_sweeps_t<4, double>(ctx) : launch_sweeps_t<8, double>(ctx);
 }
 
+// CalculateGradient: node values, then the gradient.  With `next` set (the
+// cfl of the next iteration in next_cfl), a tetrahedral context launches
+// the gradient fused with the next iteration's residual / time-step pass.
 template <int MAXF, typename R>
-bool launch_gradient_t(mfs_ctx* ctx) {
+bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0) {
     const int n = ctx->nC + ctx->nB;
     const double* q = ctx->q[ctx->cur];
     constexpr bool F64 = sizeof(R) == sizeof(double);
@@ -496,6 +499,20 @@ bool launch_gradient_t(mfs_ctx* ctx) {
                                ctx->sym_node, ctx->sym_n, qn);
             if (!check_launch(ctx, "k_node_symm")) return false;
         }
+    }
+    if (MAXF == 4 && ctx->snode && next) {
+        Timed t(ctx, "grad_flux_ts");
+        const typename dev::Real<R>::vec4* sgeo_r;
+        if constexpr (F64)
+            sgeo_r = ctx->sgeo;
+        else
+            sgeo_r = ctx->sgeo32;
+        constexpr bool MIXED = !F64;
+        hipLaunchKernelGGL((dev::k_grad_flux_ts<MIXED, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream,
+                           ctx->nC, n, ctx->snb, sgeo_r, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz,
+                           ctx->sgeo, ctx->sdn, ctx->sflip, ctx->res, ctx->dt, ctx->diag, ctx->part, ctx->K, next_cfl,
+                           0.5 * next_cfl, MIXED ? ctx->q32 : nullptr, MIXED ? ctx->diag32 : nullptr);
+        return check_launch(ctx, "k_grad_flux_ts");
     }
     {
         Timed t(ctx, "gradient");
@@ -530,11 +547,15 @@ bool launch_ghosts(mfs_ctx* ctx) {
     return check_launch(ctx, "k_ghost");
 }
 
-bool launch_gradient(mfs_ctx* ctx) {
+bool launch_gradient(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0) {
     if (ctx->precision == MFS_MIXED)
-        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx) : launch_gradient_t<6, float>(ctx);
-    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx) : launch_gradient_t<6, double>(ctx);
+        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx, next, next_cfl)
+                              : launch_gradient_t<6, float>(ctx, next, next_cfl);
+    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx, next, next_cfl)
+                          : launch_gradient_t<6, double>(ctx, next, next_cfl);
 }
+// whether launch_gradient(ctx, true, ...) also runs the next residual pass
+bool gradient_fuses_next(const mfs_ctx* ctx) { return ctx->maxf == 4 && ctx->snode; }
 
 bool usable(mfs_ctx* ctx) {
     if (!ctx) {
@@ -922,21 +943,24 @@ extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
         return MFS_ERR_ARG;
     }
     if (!ensure_hist(ctx, ctx->hist_n + nsteps)) return MFS_ERR_HIP;
+    const bool fuse = gradient_fuses_next(ctx);
     for (int s = 0; s < nsteps; ++s) {
         const double cfl = cfl_of_iter(ctx->P, ctx->iter_done);
         const double cfl_min = 0.5 * cfl;
         ctx->cfl_min = cfl_min;
         // ZeroResiduals + InviscidFlux + ComputeTimeStep with the LU-SGS
-        // diagonal, in one pass over the faces (res written from zero).
-        if (!launch_flux_ts(ctx, cfl, cfl_min)) return MFS_ERR_HIP;
+        // diagonal, in one pass over the faces (res written from zero);
+        // after the first iteration the previous gradient launch did it.
+        if ((s == 0 || !fuse) && !launch_flux_ts(ctx, cfl, cfl_min)) return MFS_ERR_HIP;
         if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
         if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
         // ForwardLUSGS: sweeps + update
         if (!launch_sweeps(ctx)) return MFS_ERR_HIP;
         // SetGhostVariables, when enabled (main.cpp:283 has the call commented out)
         if (ctx->ghost_refresh && !launch_ghosts(ctx)) return MFS_ERR_HIP;
-        // CalculateGradient
-        if (!launch_gradient(ctx)) return MFS_ERR_HIP;
+        // CalculateGradient (with the next iteration's first pass)
+        const bool next = fuse && s + 1 < nsteps;
+        if (!launch_gradient(ctx, next, next ? cfl_of_iter(ctx->P, ctx->iter_done + 1) : 0.0)) return MFS_ERR_HIP;
         ctx->hist_n++;
         ctx->iter_done++;
     }

@@ -909,6 +909,31 @@ __global__ __launch_bounds__(kBlock) void k_grad_tet(int nC, int n, const int* _
                      gz);
 }
 
+// The gradient of one iteration and the residual / time step / Diag pass of
+// the next, on tetrahedral meshes, in one launch.  Both read the cell's
+// slot neighbours and geometry, its state and volume; the second read of
+// each comes from cache instead of HBM.  The gradient feeds nothing the
+// next iteration computes, and the two write disjoint arrays, so every
+// value is the same as with two launches.
+template <bool MIXED, typename R>
+__global__ __launch_bounds__(kBlock) void k_grad_flux_ts(int nC, int n, const int* __restrict__ snb,
+                                                        const typename Real<R>::vec4* __restrict__ sgeo_r,
+                                                        const int* __restrict__ snode, const int* __restrict__ depmask,
+                                                        const double* __restrict__ q, const R* __restrict__ qn,
+                                                        const double* __restrict__ vol, R* __restrict__ gx,
+                                                        R* __restrict__ gy, R* __restrict__ gz,
+                                                        const double4* __restrict__ sgeo, const double* __restrict__ sdn,
+                                                        const int* __restrict__ sflip, double* __restrict__ res,
+                                                        double* __restrict__ dt, double* __restrict__ diag,
+                                                        double* __restrict__ part, FlowConst K, double cfl,
+                                                        double cfl_min, float* __restrict__ q32,
+                                                        float* __restrict__ diag32) {
+    const int c = xcd_block() * kBlock + (int)threadIdx.x;
+    grad_tet_body<R>(c, nC, n, snb, sgeo_r, snode, depmask, q, qn, vol, gx, gy, gz);  // no-op past nC
+    flux_ts_slot_body<4, MIXED>(c, nC, snb, sgeo, sdn, sflip, q, vol, res, dt, diag,
+                                part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
+}
+
 template <int MAXF, typename R>
 __global__ __launch_bounds__(kBlock) void k_grad(int nC, int n, const int* __restrict__ cfo,
                                                  const int* __restrict__ cf, const int2* __restrict__ fcell,
