@@ -236,7 +236,8 @@ int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
  * group is bracketed with HIP events on the context stream. */
 int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
 /* names: "flux","timestep","flux_ts" (both fused, mfs_step),"reduce","diag","clamp",
- * "fwd_sweep","bwd_sweep","ghosts","node","gradient";
+ * "fwd_sweep","bwd_sweep","ghosts","node","gradient","grad_flux_ts" (mfs_step on
+ * tetrahedra: a gradient with the next iteration's flux_ts);
  * returns accumulated milliseconds and launch count. */
 int mfs_ctx_get_timing(mfs_ctx* ctx, const char* name, double* ms, int* launches);
 int mfs_ctx_reset_timing(mfs_ctx* ctx);

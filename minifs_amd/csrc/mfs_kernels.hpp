@@ -910,11 +910,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_tet(int nC, int n, const int* _
 }
 
 // The gradient of one iteration and the residual / time step / Diag pass of
-// the next, on tetrahedral meshes, in one launch.  Both read the cell's
-// slot neighbours and geometry, its state and volume; the second read of
-// each comes from cache instead of HBM.  The gradient feeds nothing the
-// next iteration computes, and the two write disjoint arrays, so every
-// value is the same as with two launches.
+// the next, on tetrahedral meshes, in one launch.  One launch instead of
+// two lets each wave overlap the gradient's node gathers with the flux
+// pass's neighbour gathers (the HBM bytes stay those of the two passes: the
+// gathers in between evict the shared slot data).  The gradient feeds
+// nothing the next iteration computes, and the two write disjoint arrays,
+// so every value is the same as with two launches.
 template <bool MIXED, typename R>
 __global__ __launch_bounds__(kBlock) void k_grad_flux_ts(int nC, int n, const int* __restrict__ snb,
                                                         const typename Real<R>::vec4* __restrict__ sgeo_r,

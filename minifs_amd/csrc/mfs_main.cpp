@@ -160,7 +160,7 @@ int main(int argc, char** argv) {
     const double* last = h.data() + 7 * (size_t)(nh > 0 ? nh - 1 : 0);
 
     static const char* kNames[] = {"flux", "timestep", "flux_ts", "reduce", "diag",
-                                   "clamp", "fwd_sweep", "bwd_sweep", "node", "gradient"};
+                                   "clamp", "fwd_sweep", "bwd_sweep", "node", "gradient", "grad_flux_ts", "ghosts"};
     if (a.json) {
         std::printf("{\"cells\": %d, \"faces\": %d, \"bfaces\": %d, \"nodes\": %d, \"precision\": \"%s\", "
                     "\"steps\": %d, \"ms_per_step\": %.4f, \"M_cell_iters_per_s\": %.3f, \"setup_s\": %.2f, "
