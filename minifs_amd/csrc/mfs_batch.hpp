@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_b_sweep(const BatchSolve<R>* __
         const int2 su = items[it];
         const SweepArgs<R>& A = T[su.x].sw;
         if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-            (void)sweep_chunk<FWD, LPC, R>(A, T[su.x].K, FWD ? su.y : A.nchunks - 1 - su.y);
+            (void)sweep_chunk<FWD, LPC, R, false>(A, T[su.x].K, FWD ? su.y : A.nchunks - 1 - su.y);
         tk = __shfl(next, 0, 64);
     }
 }

@@ -174,7 +174,7 @@ using FacePre = std::conditional_t<sizeof(R) == sizeof(double), FluxPre<R>, Flux
 // dependency was already published.  The DQ-independent half of every face
 // term is evaluated before waiting, so only the DQ-dependent half, the
 // sum, the limiter and the stores follow the dependency's arrival.
-template <bool FWD, int LPC, typename R>
+template <bool FWD, int LPC, typename R, bool THIN = true>
 __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowConst& K, int u) {
     using B = typename Real<R>::bits;
     constexpr B kSent = Real<R>::kSent;
@@ -187,7 +187,9 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     B rv = 0;
     if (lane == 0 && rep >= 0) rv = ld_dev<R>(poll + (size_t)rep * 5);
     const bool active = k < (rec.y & 0xff);
-    const bool thin = (rec.y & 0x100) != 0;  // a thin level: every lane polls its own dependency
+    // a thin level: every lane polls its own dependency (a batch interleaves
+    // many solves, so no level of one solve is thin there: THIN = false)
+    const bool thin = THIN && (rec.y & 0x100) != 0;
     const int c = rec.x + (active ? k : 0);  // idle lanes read the first cell and store nothing
     const unsigned m = (unsigned)A.depmask[c];
     unsigned kind = FWD ? (m & 0xffu) : ((m >> 8) & 0xffu);  // slots that take part
