@@ -120,67 +120,71 @@ class Server:
         used = 0.0                  # device bytes reserved or held
         done = []
         peak_live = 0
-        while len(done) < len(reqs):
-            t = now()
-            while nxt < len(reqs) and reqs[nxt].arrival <= t:
-                building[setup_pool.submit(_setup, reqs[nxt], self.cfl, self.tree, self.device_metrics)] = reqs[nxt]
-                nxt += 1
-            for fut in [f for f in building if f.done()]:
-                r = building.pop(fut)
-                g, p = fut.result()
-                r.t_ready = now()
-                ready.append((r, g, p))
-            ready.sort(key=lambda e: e[0].arrival)
-            # reserve memory in arrival order while the budget allows; the
-            # context is built on a worker while the batch steps
-            while ready:
-                r, g, p = ready[0]
-                need = g.info["nTCell"] * self.bytes_per_cell
-                if (live or creating) and used + need > self.budget:
-                    break
-                ready.pop(0)
-                used += need
-                creating[ctx_pool.submit(Solver, g, p, self.precision)] = (r, g, need)
-            # built contexts join the batch (cheap: no host layout work here)
-            for fut in [f for f in creating if f.done()]:
-                r, g, need = creating.pop(fut)
-                s = fut.result()
-                nc = g.info["nTCell"]
-                dev_bytes = s.info()["device_bytes"]
-                self.bytes_per_cell = max(self.bytes_per_cell, dev_bytes / nc)
-                used += dev_bytes - need
-                m = batch.attach(s)
-                r.t_admit = now()
-                r.n_cells = nc
-                live[m.slot] = (r, m, g)
-            peak_live = max(peak_live, len(live))
-            if live:
-                batch.step(self.check_every)
-                batch.sync()
-                for slot, (r, m, g) in list(live.items()):
-                    h = m.history()
-                    if _converged(h, self.drop) or len(h) >= self.max_iters:
-                        r.t_done = now()
-                        r.iters = len(h)
-                        r.drop = float(h[-1, 0] / h[0, 0])
-                        r.converged = _converged(h, self.drop)
-                        r.rho_sum = float(np.sum(m.field("rho")))
-                        used -= m.info()["device_bytes"]
-                        batch.remove(m)
-                        g.close()
-                        del live[slot]
-                        done.append(r)
-                        if progress:
-                            progress(r)
-            elif building or creating:
-                # nothing on the device: wait for the next set-up or context
-                cf.wait(list(building) + list(creating), timeout=0.05, return_when=cf.FIRST_COMPLETED)
-            elif nxt < len(reqs):
-                time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
-        wall = now()
-        setup_pool.shutdown()
-        ctx_pool.shutdown()
-        batch.close()
+        try:
+            while len(done) < len(reqs):
+                t = now()
+                while nxt < len(reqs) and reqs[nxt].arrival <= t:
+                    building[setup_pool.submit(_setup, reqs[nxt], self.cfl, self.tree, self.device_metrics)] = reqs[nxt]
+                    nxt += 1
+                for fut in [f for f in building if f.done()]:
+                    r = building.pop(fut)
+                    g, p = fut.result()
+                    r.t_ready = now()
+                    ready.append((r, g, p))
+                ready.sort(key=lambda e: e[0].arrival)
+                # reserve memory in arrival order while the budget allows; the
+                # context is built on a worker while the batch steps
+                while ready:
+                    r, g, p = ready[0]
+                    need = g.info["nTCell"] * self.bytes_per_cell
+                    if (live or creating) and used + need > self.budget:
+                        break
+                    ready.pop(0)
+                    used += need
+                    creating[ctx_pool.submit(Solver, g, p, self.precision)] = (r, g, need)
+                # built contexts join the batch (cheap: no host layout work here)
+                for fut in [f for f in creating if f.done()]:
+                    r, g, need = creating.pop(fut)
+                    s = fut.result()
+                    nc = g.info["nTCell"]
+                    dev_bytes = s.info()["device_bytes"]
+                    self.bytes_per_cell = max(self.bytes_per_cell, dev_bytes / nc)
+                    used += dev_bytes - need
+                    m = batch.attach(s)
+                    r.t_admit = now()
+                    r.n_cells = nc
+                    live[m.slot] = (r, m, g)
+                peak_live = max(peak_live, len(live))
+                if live:
+                    batch.step(self.check_every)
+                    batch.sync()
+                    for slot, (r, m, g) in list(live.items()):
+                        h = m.history()
+                        if _converged(h, self.drop) or len(h) >= self.max_iters:
+                            r.t_done = now()
+                            r.iters = len(h)
+                            r.drop = float(h[-1, 0] / h[0, 0])
+                            r.converged = _converged(h, self.drop)
+                            r.rho_sum = float(np.sum(m.field("rho")))
+                            used -= m.info()["device_bytes"]
+                            batch.remove(m)
+                            g.close()
+                            del live[slot]
+                            done.append(r)
+                            if progress:
+                                progress(r)
+                elif building or creating:
+                    # nothing on the device: wait for the next set-up or context
+                    cf.wait(list(building) + list(creating), timeout=0.05, return_when=cf.FIRST_COMPLETED)
+                elif nxt < len(reqs):
+                    time.sleep(max(0.0, min(0.05, reqs[nxt].arrival - now())))
+            wall = now()
+        finally:
+            # also when a set-up or a context fails: no worker outlives the
+            # run, and the batch frees the device memory of its solves
+            setup_pool.shutdown(wait=True, cancel_futures=True)
+            ctx_pool.shutdown(wait=True, cancel_futures=True)
+            batch.close()
         return summarize(done, wall, peak_live)
 
 
