@@ -130,3 +130,27 @@ __global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restri
         grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
                            S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
 }
+
+// The gradient of iteration i (table T) fused with the residual / time-step
+// pass of iteration i+1 (table T1), as in the single-solve step: the same
+// solves in both tables, the gradient feeding nothing of iteration i+1.
+template <int MAXF, bool MIXED, typename R>
+__global__ __launch_bounds__(kBlock) void k_b_grad_flux_ts(const BatchSolve<R>* __restrict__ T,
+                                                          const BatchSolve<R>* __restrict__ T1,
+                                                          const int* __restrict__ cell_solve) {
+    const int a = cell_solve[blockIdx.x];
+    const BatchSolve<R>& S = T[a];
+    const BatchSolve<R>& S1 = T1[a];
+    const int lb = (int)blockIdx.x - S.blk0_cell;
+    const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
+    const int c = lg * kBlock + (int)threadIdx.x;
+    if (MAXF == 4 && S.snode)
+        grad_tet_body<R>(c, S.nC, S.n, S.sw.snb, S.sw.sgeo, S.snode, S.sw.depmask, S.qnew, S.qn, S.vol, S.gx, S.gy,
+                         S.gz);
+    else
+        grad_body<MAXF, R>(c, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn, S.btype, S.qnew, S.qn, S.vol,
+                           S.gx, S.gy, S.gz);
+    flux_ts_slot_body<MAXF == 4 ? 4 : 8, MIXED>(c, S1.nC, S1.sw.snb, S1.sgeo64, S1.sdn, S1.sflip, S1.q, S1.vol,
+                                                S1.res, S1.dt, S1.diag, S1.part + (size_t)lb * 8, S1.K, S1.cfl,
+                                                S1.cfl_min, S1.q32, S1.diag32);
+}

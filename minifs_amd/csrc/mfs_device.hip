@@ -1449,8 +1449,10 @@ int batch_step_t(mfs_batch* b, int nsteps) {
     for (int i = 0; i < nsteps; ++i) {
         const dev::BatchSolve<R>* Tf = D + (size_t)i * 2 * ns;
         const dev::BatchSolve<R>* Tb = Tf + ns;
-        hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
-                           b->cell_solve);
+        // after the first iteration the previous gradient launch did this pass
+        if (i == 0)
+            hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream,
+                               Tf, b->cell_solve);
         hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(dev::kReduceBlock), 0, b->stream, Tf);
         hipLaunchKernelGGL((dev::k_b_clamp<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                            b->cell_solve);
@@ -1491,8 +1493,12 @@ int batch_step_t(mfs_batch* b, int nsteps) {
                 hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(c->nsym)), dim3(kBlock), 0, b->stream, c->nsym,
                                    c->sym_node, c->sym_n, MIXED ? (R*)c->qn32 : (R*)c->qn);
         }
-        hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
-                           b->cell_solve);
+        if (i + 1 < nsteps)  // the gradient with the next iteration's residual / time-step pass
+            hipLaunchKernelGGL((dev::k_b_grad_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0,
+                               b->stream, Tf, Tf + 2 * (size_t)ns, b->cell_solve);
+        else
+            hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
+                               b->cell_solve);
         if (!batch_ok(b, hipGetLastError(), "batch gradient")) return MFS_ERR_HIP;
     }
     for (int a = 0; a < ns; ++a) {
