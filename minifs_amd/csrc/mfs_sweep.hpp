@@ -209,6 +209,10 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     } else {
         load5r(A.dqf + (size_t)c * 5, D);
     }
+    // the FP64 state the backward sweep updates (mixed precision), fetched
+    // before the wait so that no memory round trip follows the dependencies
+    double q64c[5];
+    if constexpr (!FWD && sizeof(R) != sizeof(double)) load5(A.q64 + (size_t)c * 5, q64c);
     bool rep_seen = (rep < 0) || (__shfl(rv, 0, 64) != kSent);
     R qnb[5] = {R(0), R(0), R(0), R(0), R(0)};
     R dv[5] = {R(0), R(0), R(0), R(0), R(0)};
@@ -315,7 +319,8 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
 #pragma unroll
             for (int j = 0; j < 5; ++j) qnew[j] = qc[j];
         } else {
-            load5(A.q64 + (size_t)c * 5, qnew);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) qnew[j] = q64c[j];
         }
 #pragma unroll
         for (int j = 0; j < 5; ++j) dd[j] = (double)D[j];
