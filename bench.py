@@ -503,7 +503,7 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, 5)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         return 2
     conv = args.mode == "converge"
