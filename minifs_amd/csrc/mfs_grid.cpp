@@ -560,10 +560,15 @@ void set_ghost_variables(Grid& g) {
 std::string Grid::preprocess(const mfs_params& prm, bool device_metrics) {
     if (preprocessed) return "grid already preprocessed";
     params = prm;
+    // SpecifyBC renames the boundary faces' second cells (patch -> ghost);
+    // a failure after it puts them back, so that the grid can be retried
+    const std::vector<int> f2c_boundary(m.f2c.begin(), m.f2c.begin() + 2 * (size_t)m.nB);
     std::string e = specify_bc(*this);
-    if (!e.empty()) return e;
-    e = device_metrics ? compute_metrics_device(*this) : compute_metrics(*this);
-    if (!e.empty()) return e;
+    if (e.empty()) e = device_metrics ? compute_metrics_device(*this) : compute_metrics(*this);
+    if (!e.empty()) {
+        std::copy(f2c_boundary.begin(), f2c_boundary.end(), m.f2c.begin());
+        return e;
+    }
     init_flowfield(*this);
     set_ghost_variables(*this);
     preprocessed = true;

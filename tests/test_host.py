@@ -284,9 +284,13 @@ def test_device_preprocess_without_gpu_fails_loudly():
     if gpu_available():
         pytest.skip("a GPU is present")
     g = mfs.Grid.box_bump(2).decouple()
+    f2c = g.array("f2c").copy()
     with pytest.raises(mfs.MfsError):
         g.preprocess(mfs.make_params(0.5, 0.0, 10.0), device=True)
     assert g.info["preprocessed"] == 0
+    np.testing.assert_array_equal(g.array("f2c"), f2c)  # the failed call left the grid as it was
+    g.preprocess(mfs.make_params(0.5, 0.0, 10.0))  # and the host path still works on it
+    assert g.info["preprocessed"] == 1
     g.close()
 
 
