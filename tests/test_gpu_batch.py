@@ -144,3 +144,30 @@ def test_attach_rejects_another_precision():
     s.close()
     assert len(b) == 0
     b.close()
+
+
+@pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
+def test_batch_with_symmetry_and_interface_patches(tmp_path, precision):
+    """Solves whose patches are remapped to SYMM and INTERFACE (their
+    symmetry-plane node projection runs per solve in the batch) next to a
+    plain one: the same bits as each alone."""
+    print(f"[batch] symm/interface precision={precision}", flush=True)
+    cases = []
+    for k, bc_text in enumerate(("2\n1 3\n2 4\n", "2\n1 4\n2 10\n", None)):
+        g0 = mfs.Grid.box_bump(6 + k, seed=40 + k).decouple()
+        p = mfs.make_params(mach=0.6, alpha=1.5, cfl=10.0)
+        if bc_text is None:
+            cases.append((g0.preprocess(p), p))
+            continue
+        mesh, bc = str(tmp_path / f"m{k}.mfl"), str(tmp_path / f"m{k}.bc")
+        g0.save(mesh, bc)
+        g0.close()
+        open(bc, "w").write(bc_text)
+        cases.append((mfs.Grid.load(mesh, bc).preprocess(p), p))
+    b = mfs.Batch(precision)
+    members = [b.add(g, p) for g, p in cases]
+    b.step(6)
+    for m, (g, p) in zip(members, cases):
+        ref, href = _alone(g, p, 6, precision)
+        _same(m, ref, href)
+    b.close()
