@@ -130,6 +130,30 @@ def test_cfl_ramp_bitwise_vs_reference(tmp_path, ref_driver):
     s.close()
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 1), (1, 2, 7), (9, 2, 3)])
+def test_tiny_and_ragged_boxes_bitwise_vs_reference(tmp_path, ref_driver, dims):
+    """Meshes of 6, 84 and 324 cells: a single chunk, chunks shorter than a
+    wave, one-cell-thick boxes; the whole path against the reference build."""
+    print(f"[parity] box {dims}", flush=True)
+    mach, alpha, cfl, iters = 0.5, 2.0, 10.0, 12
+    g = mfs.Grid.box_bump(*dims, seed=8).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    out = str(tmp_path / "ref")
+    ref = run_reference(ref_driver, mesh, bc, out, mach, alpha, cfl, iters, dumps=(iters,))
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
+    g.preprocess(p)
+    for prec in (mfs.MFS_FP64, mfs.MFS_MIXED):
+        s = mfs.Solver(g, p, precision=prec)
+        s.step(iters)
+        if prec == mfs.MFS_FP64:
+            _assert_state_equal(s, out, iters, g.info["nTCell"])
+            np.testing.assert_allclose(s.history()[:, :5], ref["history"][:, :5], rtol=HIST_RTOL, atol=0)
+        else:  # the mixed path runs and stays finite on the same meshes
+            assert np.isfinite(s.history()).all()
+        s.close()
+
+
 @pytest.mark.parametrize("cells", ["hex", "prism"])
 def test_polyhedral_cells_bitwise_vs_reference(tmp_path, ref_driver, cells):
     """Hexahedra (6 quadrilateral faces) and prisms (5 faces, mixed
