@@ -117,11 +117,13 @@ int main(int argc, char** argv) {
     // iterations (compute_timestep.cpp:868-871), cfl_min = 0.5 cfl_start as
     // in para_field_global.cpp:58.
     bool ghost_refresh = false;
+    bool dump_geometry = true;  // "nogeo": skip the set-up geometry dumps (large meshes)
     double ramp_start = 0.0;
     int ramp_nstep = 0;
     for (int a = 9; a < argc; ++a) {
         const std::string s = argv[a];
         if (s == "ghosts") ghost_refresh = true;
+        if (s == "nogeo") dump_geometry = false;
         if (s.rfind("ramp=", 0) == 0) {
             ramp_start = atof(s.substr(5).c_str());
             ramp_nstep = atoi(s.substr(s.find(',') + 1).c_str());
@@ -157,7 +159,7 @@ int main(int argc, char** argv) {
     PreprocessGrid(grid, bc);
 
     const int nC = grid->GetNTCell(), nB = grid->GetNBFace(), nF = grid->GetNTFace(), nN = grid->GetNTNode();
-    {
+    if (dump_geometry) {
         const int n = nC + nB;
         dump(out, "geo_xfn.f64", grid->GetXfn(), sizeof(double) * nF);
         dump(out, "geo_yfn.f64", grid->GetYfn(), sizeof(double) * nF);

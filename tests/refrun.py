@@ -11,13 +11,14 @@ STATE = ("rho", "u", "v", "w", "p")
 
 
 def run_reference(ref_driver: str, mesh: str, bc: str, out_dir: str, mach: float, alpha: float, cfl: float,
-                  iters: int, dumps=(0, 1), ghost_refresh: bool = False, ramp: tuple[float, int] | None = None) -> dict:
+                  iters: int, dumps=(0, 1), ghost_refresh: bool = False, ramp: tuple[float, int] | None = None,
+                  geometry: bool = True) -> dict:
     """ramp = (cfl_start, cfl_nstep): CFL ramp from cfl_start to `cfl`."""
     os.makedirs(out_dir, exist_ok=True)
     cmd = [ref_driver, mesh, bc, out_dir, repr(mach), repr(alpha), repr(cfl), str(iters),
            ",".join(str(d) for d in dumps)] + (["ghosts"] if ghost_refresh else []) + \
-        ([f"ramp={ramp[0]!r},{ramp[1]}"] if ramp else [])
-    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, timeout=600)
+        ([f"ramp={ramp[0]!r},{ramp[1]}"] if ramp else []) + ([] if geometry else ["nogeo"])
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, timeout=900)
     return load_dir(out_dir, iters)
 
 

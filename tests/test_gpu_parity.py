@@ -154,6 +154,29 @@ def test_tiny_and_ragged_boxes_bitwise_vs_reference(tmp_path, ref_driver, dims):
         s.close()
 
 
+def test_full_size_bitwise_vs_reference(tmp_path, ref_driver):
+    """The 100^3 x 6 = 6.0M-cell box+bump mesh (the throughput workload),
+    two iterations: state, residual, time step, DQ and gradients bitwise
+    against the reference's serial build on the same mesh."""
+    print("[parity] 6.0M cells, 2 iterations (the reference build takes about a minute)", flush=True)
+    mach, alpha, cfl, iters = 0.5, 0.0, 10.0, 2
+    g = mfs.Grid.box_bump(100).decouple()
+    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
+    g.save(mesh, bc)
+    out = str(tmp_path / "ref")
+    ref = run_reference(ref_driver, mesh, bc, out, mach, alpha, cfl, iters, dumps=(iters,), geometry=False)
+    p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
+    g.preprocess(p, device=True)
+    s = mfs.Solver(g, p)
+    s.step(iters)
+    _assert_state_equal(s, out, iters, g.info["nTCell"])
+    h = s.history()
+    np.testing.assert_array_equal(h[:, 5:], ref["history"][:, 5:])
+    np.testing.assert_allclose(h[:, :5], ref["history"][:, :5], rtol=HIST_RTOL, atol=0)
+    s.close()
+    g.close()
+
+
 @pytest.mark.parametrize("cells", ["hex", "prism"])
 def test_polyhedral_cells_bitwise_vs_reference(tmp_path, ref_driver, cells):
     """Hexahedra (6 quadrilateral faces) and prisms (5 faces, mixed
