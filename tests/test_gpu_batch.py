@@ -171,3 +171,22 @@ def test_batch_with_symmetry_and_interface_patches(tmp_path, precision):
         ref, href = _alone(g, p, 6, precision)
         _same(m, ref, href)
     b.close()
+
+
+@pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
+def test_batch_of_large_solves_equals_standalone(precision):
+    """Solves of 94k, 162k and 384k cells (thousands of chunks each, long
+    interleaved item lists, many XCD groups per solve): the same bits as
+    alone after a few fused iterations."""
+    print(f"[batch] large solves precision={precision}", flush=True)
+    cases = [_case(25, 0.5, 0.0, 61), _case(30, 0.8, 1.25, 62), _case(40, 0.4, -2.0, 63)]
+    b = mfs.Batch(precision)
+    members = [b.add(g, p) for g, p in cases]
+    b.step(3)
+    b.step(2)
+    for m, (g, p) in zip(members, cases):
+        ref, href = _alone(g, p, 5, precision)
+        _same(m, ref, href)
+    b.close()
+    for g, _ in cases:
+        g.close()
