@@ -31,13 +31,15 @@ struct Args {
     int steps = 20, warmup = 3;
     int precision = MFS_FP64;
     bool entry_points = false, timing = false, json = false;
+    int tree_parts = 4, tree_leaf = 512;  // TDTree parameters (CreateTree's 4 and 512)
+    bool device_metrics = false;          // ComputeMetrics on the GPU
 };
 
 int usage() {
     std::fprintf(stderr,
                  "usage: mfs_solve [--mesh FILE --bc FILE | --box N [--cells tet|hex|prism]] [--mach M] "
                  "[--alpha DEG] [--cfl C] [--steps K] [--warmup W] [--precision fp64|mixed] [--entry-points] "
-                 "[--timing] [--json]\n");
+                 "[--timing] [--json] [--tree PARTS,LEAF] [--device-metrics]\n");
     return 2;
 }
 
@@ -56,6 +58,8 @@ bool parse(int argc, char** argv, Args& a) {
             a.timing = true;
         } else if (k == "--json") {
             a.json = true;
+        } else if (k == "--device-metrics") {
+            a.device_metrics = true;
         } else if (!val(v)) {
             return false;
         } else if (k == "--mesh") {
@@ -80,6 +84,8 @@ bool parse(int argc, char** argv, Args& a) {
             a.steps = std::atoi(v);
         } else if (k == "--warmup") {
             a.warmup = std::atoi(v);
+        } else if (k == "--tree") {
+            if (std::sscanf(v, "%d,%d", &a.tree_parts, &a.tree_leaf) != 2) return false;
         } else if (k == "--precision") {
             const std::string c = v;
             if (c == "fp64") a.precision = MFS_FP64;
@@ -138,8 +144,9 @@ int main(int argc, char** argv) {
     } else if (mfs_grid_generate_box(a.box, a.box, a.box, a.cells, 0.1, 0.05, 0.1, 20260101ULL, &g) != MFS_OK) {
         return fail("box mesh");
     }
-    if (mfs_grid_decouple(g) != MFS_OK) return fail("Decoupling");
-    if (mfs_grid_preprocess(g, &p) != MFS_OK) return fail("PreprocessGrid");
+    if (mfs_grid_decouple_ex(g, a.tree_parts, a.tree_leaf) != MFS_OK) return fail("Decoupling");
+    if ((a.device_metrics ? mfs_grid_preprocess_device(g, &p) : mfs_grid_preprocess(g, &p)) != MFS_OK)
+        return fail("PreprocessGrid");
     mfs_grid_info gi;
     mfs_mesh_view view;
     if (mfs_grid_get_info(g, &gi) != MFS_OK || mfs_grid_view(g, &view) != MFS_OK) return fail("grid view");

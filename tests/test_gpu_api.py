@@ -122,24 +122,26 @@ def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles)
     b.close()
 
 
-@pytest.mark.parametrize("entry_points", [False, True])
-def test_native_driver_matches_python_api(entry_points):
+@pytest.mark.parametrize("entry_points,setup", [(False, ""), (True, ""), (False, "tree+device")])
+def test_native_driver_matches_python_api(entry_points, setup):
     """mfs_solve (the native driver of the reference's main loop) gives the
     same residual history record as the library driven from Python, through
-    mfs_step or through the kernel entry points."""
+    mfs_step or through the kernel entry points; also with another tree
+    order and the set-up metrics on the GPU."""
     import json
     import os
     import subprocess
 
     exe = os.path.join(os.path.dirname(mfs.__file__), "mfs_solve")
     cmd = [exe, "--box", "8", "--steps", "4", "--warmup", "1", "--mach", "0.8", "--alpha", "1.25", "--cfl", "10",
-           "--json"] + (["--entry-points"] if entry_points else [])
+           "--json"] + (["--entry-points"] if entry_points else []) + \
+        (["--tree", "2,64", "--device-metrics"] if setup else [])
     print("[native]", " ".join(cmd), flush=True)
     out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
     p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
-    g = mfs.Grid.box_bump(8).decouple().preprocess(p)
+    g = mfs.Grid.box_bump(8).decouple(*((2, 64) if setup else (4, 512))).preprocess(p)
     s = mfs.Solver(g, p)
     s.step(5)
     h = s.history()
