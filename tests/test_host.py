@@ -323,3 +323,14 @@ def test_preprocess_rejects_unknown_patch(tmp_path):
     h = mfs.Grid.load(mesh, bc)
     with pytest.raises(mfs.MfsError):
         h.preprocess(mfs.make_params(0.5, 0.0, 10.0))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the library measures one GPU's share; a request for several exits
+    # with a message before any device work
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+    assert r.stdout.strip() == ""  # no JSON line
