@@ -341,19 +341,11 @@ def mode_batch_converge(args) -> dict:
     residual drop of --batch-drop in one device batch (converged solves
     leave it), against the same solves run one at a time on the plain FP64
     path."""
-    import numpy as np
-
     import minifs_amd as mfs
-    from minifs_amd.serve import Request, Server, run_sequential
+    from minifs_amd.serve import Server, make_batch_requests, run_sequential
 
     def reqs():
-        r = np.random.default_rng(args.batch_seed)
-        out = []
-        for i in range(args.solves):
-            nx, ny, nz = (int(r.integers(args.batch_lo, args.batch_hi + 1)) for _ in range(3))
-            bh, mach, alpha = float(r.uniform(0.02, 0.08)), float(r.uniform(0.3, 0.85)), float(r.uniform(-3.0, 5.0))
-            out.append(Request(i, 0.0, nx, ny, nz, mach, alpha, bh, int(r.integers(1, 2**31))))
-        return out
+        return make_batch_requests(args.solves, args.batch_lo, args.batch_hi, args.batch_seed)
 
     prec = mfs.MFS_MIXED if args.precision == "mixed" else mfs.MFS_FP64
     batch_reqs = reqs()
@@ -448,7 +440,7 @@ def mode_serve(args) -> dict:
                               tree=args.tree,
                               progress=lambda r: print(f"[sequential] request {r.rid} {r.n_cells} cells {r.iters} it "
                                                        f"{r.latency:.2f} s", file=sys.stderr, flush=True))
-        same = all(a.rho_sum == b.rho_sum and a.iters == b.iters
+        same = all(a.state_sha256 == b.state_sha256 and a.iters == b.iters
                    for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(again, key=lambda r: r.rid))) \
             if prec == mfs.MFS_FP64 else None
         base["results_equal_to_served"] = same

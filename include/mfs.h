@@ -63,6 +63,11 @@ typedef struct mfs_params {
     double epsa_r;      /* entropy-fix constant alf_l = epsa_r */
     double lhs_omga;    /* LU-SGS lhs ratio */
     double ratio_dtmax; /* dt_max/dt_min limit */
+    /* limits of the state, para_field_global.cpp:200-208: rho in
+     * [ratio_rhop_min rho_s, ratio_rhop_max rho_stag], absolute p in
+     * [ratio_rhop_min p_bar, ratio_rhop_max (p_stag + p_bar)], CFL reduced
+     * below absolute p = ratio_p_break p_bar (defaults 10, 1e-4, 0.01) */
+    double ratio_rhop_max, ratio_rhop_min, ratio_p_break;
     /* derived by mfs_params_derive */
     double amu, ainf, uqq;
     double u_s, v_s, w_s, p_s, rho_s, p_bar, gamm1;
@@ -230,6 +235,7 @@ typedef struct mfs_ctx_info {
     int sweep_levels;      /* DAG depth of the LU-SGS sweep order */
     size_t device_bytes;   /* device memory held by the context */
     int tile_cells, sweep_waves, sweep_tiles, sweep_workgroups, sweep_by_tiles;
+    int compute_units;     /* CUs of the context's device (the sweep grid is sized on them) */
 } mfs_ctx_info;
 int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
 /* Kernel time accounting: when enabled, every launch of the named kernel

@@ -27,7 +27,8 @@ class MfsParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("mach", "alpha", "beita", "re", "T", "gam", "gascon",
                                           "cfl_start", "cfl_end")] + [
         ("cfl_nstep", C.c_int)] + [(n, C.c_double) for n in (
-            "epsa_r", "lhs_omga", "ratio_dtmax", "amu", "ainf", "uqq", "u_s", "v_s", "w_s", "p_s", "rho_s",
+            "epsa_r", "lhs_omga", "ratio_dtmax", "ratio_rhop_max", "ratio_rhop_min", "ratio_p_break", "amu",
+            "ainf", "uqq", "u_s", "v_s", "w_s", "p_s", "rho_s",
             "p_bar", "gamm1", "p_stag", "e_stag", "p_min", "p_max", "rho_min", "rho_max", "p_break",
             "e_stag_max", "alf_l", "alf_n")]
 
@@ -58,7 +59,7 @@ class MfsCtxInfo(C.Structure):
     _fields_ = [("nTCell", C.c_int), ("nTFace", C.c_int), ("nBFace", C.c_int), ("nTNode", C.c_int),
                 ("precision", C.c_int), ("sweep_levels", C.c_int), ("device_bytes", C.c_size_t),
                 ("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_tiles", C.c_int),
-                ("sweep_workgroups", C.c_int), ("sweep_by_tiles", C.c_int)]
+                ("sweep_workgroups", C.c_int), ("sweep_by_tiles", C.c_int), ("compute_units", C.c_int)]
 
 
 class MfsCtxOptions(C.Structure):

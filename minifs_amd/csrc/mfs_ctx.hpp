@@ -96,6 +96,7 @@ struct mfs_ctx {
     int nchunk = 0, lpc = 4;
     int nlev = 0;
     int sweep_grid = 0;  // resident workgroups of the sweep launch
+    int compute_units = 0;  // CUs of the device
     // reductions and history
     double* part = nullptr;  // nblk * 8: 5 squared-norm partials, dt min, dt max, spare
     double* run = nullptr;   // [0] dt_max [1] dt_min [2] clamp flag [3..7] last norms

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <sstream>
 #include <vector>
 #include <cstdio>
@@ -287,6 +288,77 @@ void collect_timing(mfs_ctx* ctx) {
 
 bool check_launch(mfs_ctx* ctx, const char* what) { return hip_ok(ctx, hipGetLastError(), what); }
 
+// Sweeps of different streams must not be in flight together on one device.
+// A sweep grid is sized to be resident on an otherwise idle device; two
+// grids that each hold part of the GPU could each wait on chunks owned by
+// waves the other keeps from starting, until their bounded waits end both
+// sweeps with an error.  So every sweep pair (forward and backward, of a
+// context or of a batch, from any host thread) is ordered after the
+// previous one launched on the same device: under a per-device lock, the
+// launching stream waits on the GPU, through an event, for the device's
+// last sweep pair, and records that event after its own.  Kernels that do
+// not wait on other waves (everything but the sweeps) may still overlap.
+// The table is allocated once and never freed: static destructors run at
+// process exit, after which the HIP runtime may be gone.
+constexpr int kMaxDevices = 64;
+struct SweepOrder {
+    std::mutex mu;
+    hipEvent_t last = nullptr;          // recorded after the device's latest sweep pair
+    hipStream_t last_stream = nullptr;  // the stream it was recorded on
+};
+SweepOrder* const g_sweep_order = new SweepOrder[kMaxDevices];
+
+// Holds the device's lock from before the sweeps are enqueued until the
+// event is recorded after them.
+class SweepTurn {
+  public:
+    SweepTurn(int device, hipStream_t stream) : o_(g_sweep_order[device & (kMaxDevices - 1)]), st_(stream), lk_(o_.mu) {
+        if (device < 0 || device >= kMaxDevices) {
+            err_ = "device id out of range for the sweep order table";
+            return;
+        }
+        if (!o_.last && hipEventCreateWithFlags(&o_.last, hipEventDisableTiming) != hipSuccess) {
+            o_.last = nullptr;
+            err_ = "hipEventCreate(sweep order)";
+            return;
+        }
+        if (o_.last_stream && o_.last_stream != st_ && hipStreamWaitEvent(st_, o_.last, 0) != hipSuccess)
+            err_ = "hipStreamWaitEvent(sweep order)";
+    }
+    // after the sweeps are enqueued
+    bool done() {
+        if (err_) return false;
+        if (hipEventRecord(o_.last, st_) != hipSuccess) {
+            err_ = "hipEventRecord(sweep order)";
+            return false;
+        }
+        o_.last_stream = st_;
+        return true;
+    }
+    const char* error() const { return err_; }
+
+  private:
+    SweepOrder& o_;
+    hipStream_t st_;
+    std::lock_guard<std::mutex> lk_;
+    const char* err_ = nullptr;
+};
+
+bool sweep_turn_failed(mfs_ctx* ctx, const char* what) {
+    set_error(what);
+    if (ctx) ctx->broken = true;
+    return false;
+}
+
+// A stream about to be destroyed leaves the table, so that a later stream
+// created at the same address still waits for the device's last sweeps.
+void forget_sweep_stream(int device, hipStream_t stream) {
+    if (device < 0 || device >= kMaxDevices) return;
+    SweepOrder& o = g_sweep_order[device];
+    std::lock_guard<std::mutex> lk(o.mu);
+    if (o.last_stream == stream) o.last_stream = reinterpret_cast<hipStream_t>(~uintptr_t(0));
+}
+
 double cfl_of_iter(const mfs_params& P, int iter_done) {
     // LimitTimeStep (compute_timestep.cpp:868-871)
     if (iter_done > P.cfl_nstep) return P.cfl_end;
@@ -431,6 +503,8 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     A.ngroups = std::min(kTicketGroups, (int)grid.x);
     int* const tk_f = ctx->tickets;
     int* const tk_b = ctx->tickets + kTicketGroups * kTicketStride;
+    SweepTurn turn(ctx->device, ctx->stream);
+    if (turn.error()) return sweep_turn_failed(ctx, turn.error());
     {
         Timed t(ctx, "fwd_sweep");
         A.tickets = tk_f;
@@ -453,6 +527,7 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
                     "k_sweep<bwd>"))
             return false;
     }
+    if (!turn.done()) return sweep_turn_failed(ctx, turn.error());
     ctx->cur = 1 - ctx->cur;
     return true;
 }
@@ -586,7 +661,10 @@ void free_all(mfs_ctx* ctx) {
     ctx->pending.clear();
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     ctx->event_pool.clear();
-    if (ctx->stream && ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->owns_stream) {
+        forget_sweep_stream(ctx->device, ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
     ctx->stream = nullptr;
 }
 
@@ -756,6 +834,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
         if (!ok) set_error("mfs_ctx_create: occupancy query failed");
         // co-resident workgroups
         ctx->sweep_grid = std::max(1, std::min(per_cu, kSweepWavesPerCU / ctx->sweep_waves) * cus);
+        ctx->compute_units = cus;
     }
     if (ok) {
         std::vector<int2> fc(L.nF);
@@ -1175,6 +1254,7 @@ extern "C" int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info) {
                                                  : std::min(ctx->sweep_grid, (ctx->nchunk + ctx->sweep_waves - 1) /
                                                                                  ctx->sweep_waves);
     info->sweep_by_tiles = ctx->sweep_by_tiles;
+    info->compute_units = ctx->compute_units;
     return MFS_OK;
 }
 
@@ -1405,8 +1485,8 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         }
         if (!ensure_hist(c, c->hist_n + nsteps)) return MFS_ERR_HIP;
     }
-    // two tables per iteration (forward and backward sweep reps differ)
-    const size_t per_iter = 2 * (size_t)ns * sizeof(dev::BatchSolve<R>);
+    // one table per iteration, read by every launch of that iteration
+    const size_t per_iter = (size_t)ns * sizeof(dev::BatchSolve<R>);
     const size_t bytes = per_iter * (size_t)nsteps;
     if (b->table_pending) {
         if (!batch_ok(b, hipEventSynchronize(b->table_copied), "batch table event")) return MFS_ERR_HIP;
@@ -1430,10 +1510,7 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         for (int a = 0; a < ns; ++a) {
             mfs_ctx* c = b->slots[b->active[a]];
             const int cur = c->cur ^ (i & 1);
-            dev::BatchSolve<R>& Ef = H[(size_t)i * 2 * ns + a];
-            fill_entry<R>(Ef, c, cur, blk_c, blk_n, c->iter_done + i, c->hist_n + i);
-            dev::BatchSolve<R>& Eb = H[(size_t)i * 2 * ns + ns + a];
-            Eb = Ef;
+            fill_entry<R>(H[(size_t)i * ns + a], c, cur, blk_c, blk_n, c->iter_done + i, c->hist_n + i);
             blk_c += c->nblk_cell;
             blk_n += (int)nblk_xcd(c->nN);
         }
@@ -1447,8 +1524,7 @@ int batch_step_t(mfs_batch* b, int nsteps) {
     const dim3 sweep_grid((unsigned)std::max(1, std::min(b->sweep_grid, (b->nitems + kBatchSweepWaves - 1) /
                                                                          kBatchSweepWaves)));
     for (int i = 0; i < nsteps; ++i) {
-        const dev::BatchSolve<R>* Tf = D + (size_t)i * 2 * ns;
-        const dev::BatchSolve<R>* Tb = Tf + ns;
+        const dev::BatchSolve<R>* Tf = D + (size_t)i * ns;
         // after the first iteration the previous gradient launch did this pass
         if (i == 0)
             hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream,
@@ -1458,6 +1534,8 @@ int batch_step_t(mfs_batch* b, int nsteps) {
                            b->cell_solve);
         if (!batch_ok(b, hipGetLastError(), "batch flux/time step")) return MFS_ERR_HIP;
         {
+            SweepTurn turn(b->device, b->stream);
+            if (turn.error()) return sweep_turn_failed(nullptr, turn.error()) ? MFS_OK : MFS_ERR_HIP;
             int n_it = b->nitems;
             int ng = std::min(kTicketGroups, (int)sweep_grid.x);
             int* tk_f = b->tickets;
@@ -1468,12 +1546,13 @@ int batch_step_t(mfs_batch* b, int nsteps) {
                                                      dim3(64 * kBatchSweepWaves), args, 0, b->stream),
                           "batch forward sweep"))
                 return MFS_ERR_HIP;
-            void* argsb[] = {(void*)&Tb, (void*)&b->items_b, (void*)&n_it, (void*)&tk_b, (void*)&tk_f, (void*)&ng};
+            void* argsb[] = {(void*)&Tf, (void*)&b->items_b, (void*)&n_it, (void*)&tk_b, (void*)&tk_f, (void*)&ng};
             if (!batch_ok(b,
                           hipLaunchKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
                                                      dim3(64 * kBatchSweepWaves), argsb, 0, b->stream),
                           "batch backward sweep"))
                 return MFS_ERR_HIP;
+            if (!turn.done()) return sweep_turn_failed(nullptr, turn.error()) ? MFS_OK : MFS_ERR_HIP;
         }
         for (int a = 0; a < ns; ++a) {  // SetGhostVariables for the solves that asked for it
             mfs_ctx* c = b->slots[b->active[a]];
@@ -1495,7 +1574,7 @@ int batch_step_t(mfs_batch* b, int nsteps) {
         }
         if (i + 1 < nsteps)  // the gradient with the next iteration's residual / time-step pass
             hipLaunchKernelGGL((dev::k_b_grad_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0,
-                               b->stream, Tf, Tf + 2 * (size_t)ns, b->cell_solve);
+                               b->stream, Tf, Tf + (size_t)ns, b->cell_solve);
         else
             hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                                b->cell_solve);
@@ -1594,6 +1673,7 @@ extern "C" int mfs_batch_attach(mfs_batch* b, mfs_ctx* c, int* slot) {
     }
     // the solve's work goes to the batch stream from now on
     if (!batch_ok(b, hipStreamSynchronize(c->stream), "mfs_batch_attach: context stream")) return MFS_ERR_HIP;
+    forget_sweep_stream(c->device, c->stream);
     (void)hipStreamDestroy(c->stream);
     c->stream = b->stream;
     c->owns_stream = false;
@@ -1699,6 +1779,7 @@ extern "C" int mfs_batch_destroy(mfs_batch* b) {
     (void)hipHostFree(b->table_host);
     (void)hipFree(b->table_dev);
     (void)hipEventDestroy(b->table_copied);
+    forget_sweep_stream(b->device, b->stream);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return MFS_OK;

@@ -27,6 +27,9 @@ void params_default(mfs_params& p) {
     p.epsa_r = 0.1;
     p.lhs_omga = 1.0;
     p.ratio_dtmax = 1.0e80;
+    p.ratio_rhop_max = 10.0;  // :200-202
+    p.ratio_rhop_min = 1.0e-4;
+    p.ratio_p_break = 0.01;
     params_derive(p);
 }
 
@@ -51,12 +54,11 @@ void params_derive(mfs_params& p) {
     const double rho_stag = pow(temp, 1.0 / p.gamm1) * p.rho_s;
     p.p_stag = (p_stag_t - 1.0) * p.p_bar;
     p.e_stag = p.p_bar / p.gamm1 + 0.5 * p.rho_s * (p.u_s * p.u_s + p.v_s * p.v_s + p.w_s * p.w_s);
-    const double ratio_rhop_max = 10.0, ratio_rhop_min = 1.0e-4, ratio_p_break = 0.01;
-    p.p_min = ratio_rhop_min * p.p_bar - p.p_bar;
-    p.p_max = ratio_rhop_max * (p.p_stag + p.p_bar) - p.p_bar;
-    p.rho_min = ratio_rhop_min * p.rho_s;
-    p.rho_max = ratio_rhop_max * rho_stag;
-    p.p_break = ratio_p_break * p.p_bar - p.p_bar;
+    p.p_min = p.ratio_rhop_min * p.p_bar - p.p_bar;
+    p.p_max = p.ratio_rhop_max * (p.p_stag + p.p_bar) - p.p_bar;
+    p.rho_min = p.ratio_rhop_min * p.rho_s;
+    p.rho_max = p.ratio_rhop_max * rho_stag;
+    p.p_break = p.ratio_p_break * p.p_bar - p.p_bar;
     p.e_stag_max = 1.0e5 * p.e_stag;
     p.alf_l = p.epsa_r;
     p.alf_n = p.epsa_r;

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import dataclasses
+import hashlib
 import math
 import time
 
@@ -51,6 +52,7 @@ class Request:
     converged: bool = False
     n_cells: int = 0
     rho_sum: float = math.nan     # checksum of the final density
+    state_sha256: str = ""        # SHA-256 of the final rho, u, v, w, p (reference layout, ghosts included)
 
     @property
     def latency(self) -> float:
@@ -74,6 +76,36 @@ def make_requests(n: int, rate: float, cells_lo: float, cells_hi: float, seed: i
         out.append(Request(i, t, nx, ny, nz, float(rng.uniform(0.3, 0.85)), float(rng.uniform(-3.0, 5.0)),
                            float(rng.uniform(0.02, 0.08)), int(rng.integers(1, 2**31))))
     return out
+
+
+def make_batch_requests(n: int, lo: int, hi: int, seed: int) -> list[Request]:
+    """n solves present at t = 0: boxes of nx, ny, nz ~ U{lo..hi} hexes x 6
+    tetrahedra, bump height ~ U[0.02, 0.08], Mach ~ U[0.3, 0.85], AoA ~
+    U[-3, 5] degrees, mesh seed drawn from the same generator."""
+    r = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx, ny, nz = (int(r.integers(lo, hi + 1)) for _ in range(3))
+        bh, mach, alpha = float(r.uniform(0.02, 0.08)), float(r.uniform(0.3, 0.85)), float(r.uniform(-3.0, 5.0))
+        out.append(Request(i, 0.0, nx, ny, nz, mach, alpha, bh, int(r.integers(1, 2**31))))
+    return out
+
+
+def state_digest(solver) -> str:
+    """SHA-256 of a solve's state arrays rho, u, v, w, p (reference layout)."""
+    h = hashlib.sha256()
+    for nm in ("rho", "u", "v", "w", "p"):
+        h.update(solver.field(nm).tobytes())
+    return h.hexdigest()
+
+
+def _finish(r: Request, solver, h: np.ndarray, drop: float, t: float) -> None:
+    r.t_done = t
+    r.iters = len(h)
+    r.drop = float(h[-1, 0] / h[0, 0])
+    r.converged = _converged(h, drop)
+    r.rho_sum = float(np.sum(solver.field("rho")))
+    r.state_sha256 = state_digest(solver)
 
 
 def _setup(r: Request, cfl: float, tree: tuple[int, int] = (4, 512), device_metrics: bool = True):
@@ -161,11 +193,7 @@ class Server:
                     for slot, (r, m, g) in list(live.items()):
                         h = m.history()
                         if _converged(h, self.drop) or len(h) >= self.max_iters:
-                            r.t_done = now()
-                            r.iters = len(h)
-                            r.drop = float(h[-1, 0] / h[0, 0])
-                            r.converged = _converged(h, self.drop)
-                            r.rho_sum = float(np.sum(m.field("rho")))
+                            _finish(r, m, h, self.drop, now())
                             used -= m.info()["device_bytes"]
                             batch.remove(m)
                             g.close()
@@ -210,11 +238,7 @@ def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int =
             h = s.history()
             if _converged(h, drop) or len(h) >= max_iters:
                 break
-        r.t_done = now()
-        r.iters = len(h)
-        r.drop = float(h[-1, 0] / h[0, 0])
-        r.converged = _converged(h, drop)
-        r.rho_sum = float(np.sum(s.field("rho")))
+        _finish(r, s, h, drop, now())
         s.close()
         g.close()
         done.append(r)

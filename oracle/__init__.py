@@ -33,6 +33,8 @@ def lib():
         L.or_set_ghost_refresh.argtypes = [C.c_void_p, C.c_int]
         L.or_set.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]
         L.or_set_ghosts.argtypes = [C.c_void_p]
+        L.or_set_limits.argtypes = [C.c_void_p] + [C.c_double] * 4
+        L.or_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.c_int, C.c_int]
         L.or_free.argtypes = [C.c_void_p]
         L.or_free.restype = None
         L.or_error.restype = C.c_char_p
@@ -40,12 +42,22 @@ def lib():
     return _lib
 
 
-class Oracle:
-    """CPU restatement run on one FlowStar mesh file."""
+# Guard-branch counter names, in the order of the or_counts enum (mfs_oracle.h).
+COUNTS = ("fwd_limit1", "fwd_limit2", "fwd_limit3", "bwd_limit1", "bwd_limit2", "bwd_limit3",
+          "relax_rho", "relax_p", "clamp_rho_lo", "clamp_rho_hi", "clamp_p_lo", "clamp_p_hi",
+          "cfl_ramp", "cfl_pmin", "cfl_det", "dtmax", "ghost_rho_lo", "ghost_rho_hi", "ghost_p_lo", "ghost_p_hi")
 
-    def __init__(self, mesh: str, bc: str, mach: float, alpha: float, cfl: float):
+
+class Oracle:
+    """CPU restatement run on one FlowStar mesh file.  limits = (ratio_rhop_min,
+    ratio_rhop_max, ratio_p_break, ratio_dtmax) replaces the reference's
+    defaults (1e-4, 10, 0.01, 1e80) before the set-up."""
+
+    def __init__(self, mesh: str, bc: str, mach: float, alpha: float, cfl: float, limits=None):
         self._h = C.c_void_p()
         self._check(lib().or_load(mesh.encode(), bc.encode(), C.byref(self._h)))
+        if limits is not None:
+            self._check(lib().or_set_limits(self._h, *[float(x) for x in limits]))
         self._check(lib().or_preprocess(self._h, mach, alpha, cfl))
         s = [C.c_int() for _ in range(4)]
         lib().or_sizes(self._h, *[C.byref(x) for x in s])
@@ -82,6 +94,12 @@ class Oracle:
 
     def set_ghosts(self):
         self._check(lib().or_set_ghosts(self._h))
+
+    def counts(self, reset: bool = False) -> dict:
+        """How often each guard branch was taken since the set-up or the last reset."""
+        out = (C.c_longlong * len(COUNTS))()
+        self._check(lib().or_counts(self._h, out, len(COUNTS), 1 if reset else 0))
+        return dict(zip(COUNTS, out))
 
     def close(self):
         if self._h:

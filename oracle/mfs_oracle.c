@@ -31,6 +31,7 @@ static double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min *
 
 typedef struct {
     double mach, alpha, re, T, gam, gascon, cfl_start, cfl_end, cfl_ratio, epsa_r, lhs_omga, ratio_dtmax;
+    double ratio_rhop_max, ratio_rhop_min, ratio_p_break; /* para_field_global.cpp:200-202 */
     int cfl_nstep;
     double u_s, v_s, w_s, p_s, rho_s, p_bar, gamm1, p_min, p_max, rho_min, rho_max, p_break, alf_l;
 } Par;
@@ -57,8 +58,17 @@ struct or_grid {
     int iter_done;
     double dt_max, dt_min, cfl_min;
     int ghost_refresh; /* SetGhostVariables after each update (the commented main.cpp:283 call) */
+    double lim[4];     /* ratio_rhop_min, ratio_rhop_max, ratio_p_break, ratio_dtmax (or_set_limits) */
+    long long cnt[OR_NCOUNT]; /* how often each guard branch was taken (or_counts) */
     Par P;
 };
+
+/* counts a guard branch; the sweeps and the update run in parallel loops in
+ * the threaded mode */
+static void bump(or_grid* g, int k) {
+#pragma omp atomic
+    g->cnt[k]++;
+}
 
 /* ---------------------------------------------------------------- reader */
 /* ReadMFlowGrid_binary (load_grid.cpp:129-273) + ReadMFlowBoundaryCondition (:282-314) */
@@ -118,13 +128,18 @@ int or_load(const char* mesh_path, const char* bc_path, or_grid** out) {
         g->patch_type[i] = type;
     }
     fclose(bf);
+    g->lim[0] = 1.0e-4; /* para_field_global.cpp:200-202, 115 */
+    g->lim[1] = 10.0;
+    g->lim[2] = 0.01;
+    g->lim[3] = 1.0e80;
     *out = g;
     return 0;
 }
 
 /* ---------------------------------------------------------------- set-up */
-static void set_par(Par* P, double mach, double alpha, double cfl) {
-    /* para_field_global.cpp:37-209 */
+static void set_par(Par* P, double mach, double alpha, double cfl, const double lim[4]) {
+    /* para_field_global.cpp:37-209; lim = ratio_rhop_min, ratio_rhop_max,
+     * ratio_p_break, ratio_dtmax (the reference's defaults unless set) */
     P->mach = mach;
     P->alpha = alpha;
     P->re = 1.8140449177e7;
@@ -138,7 +153,10 @@ static void set_par(Par* P, double mach, double alpha, double cfl) {
     P->epsa_r = 0.1;
     P->alf_l = P->epsa_r;
     P->lhs_omga = 1.0;
-    P->ratio_dtmax = 1.0e80;
+    P->ratio_rhop_min = lim ? lim[0] : 1.0e-4;
+    P->ratio_rhop_max = lim ? lim[1] : 10.0;
+    P->ratio_p_break = lim ? lim[2] : 0.01;
+    P->ratio_dtmax = lim ? lim[3] : 1.0e80;
     const double tref = 288.15, sref = 110.4, amuref = 1.78938e-5;
     double trat = (P->T / tref);
     double amu = amuref * trat * sqrt(trat) * (tref + sref) / (P->T + sref);
@@ -156,11 +174,11 @@ static void set_par(Par* P, double mach, double alpha, double cfl) {
     double p_stag_t = pow(temp, P->gam / P->gamm1);
     double rho_stag = pow(temp, 1.0 / P->gamm1) * P->rho_s;
     double p_stag = (p_stag_t - 1.0) * P->p_bar;
-    P->p_min = 1.0e-4 * P->p_bar - P->p_bar;
-    P->p_max = 10.0 * (p_stag + P->p_bar) - P->p_bar;
-    P->rho_min = 1.0e-4 * P->rho_s;
-    P->rho_max = 10.0 * rho_stag;
-    P->p_break = 0.01 * P->p_bar - P->p_bar;
+    P->p_min = P->ratio_rhop_min * P->p_bar - P->p_bar;
+    P->p_max = P->ratio_rhop_max * (p_stag + P->p_bar) - P->p_bar;
+    P->rho_min = P->ratio_rhop_min * P->rho_s;
+    P->rho_max = P->ratio_rhop_max * rho_stag;
+    P->p_break = P->ratio_p_break * P->p_bar - P->p_bar;
 }
 
 static int metrics(or_grid* g) {
@@ -447,8 +465,12 @@ static void ghosts(or_grid* g) {
             p[c2] = 2 * p[c2] - pp;
             p[c2] -= p_bar;
         }
+        if (rho[c2] < P->rho_min) bump(g, OR_GHOST_RHO_LO);
+        if (rho[c2] > P->rho_max) bump(g, OR_GHOST_RHO_HI);
         rho[c2] = dmax(rho[c2], P->rho_min);
         rho[c2] = dmin(rho[c2], P->rho_max);
+        if (p[c2] < P->p_min) bump(g, OR_GHOST_P_LO);
+        if (p[c2] > P->p_max) bump(g, OR_GHOST_P_HI);
         p[c2] = dmax(p[c2], P->p_min);
         p[c2] = dmin(p[c2], P->p_max);
     }
@@ -583,7 +605,7 @@ static void connectivity_and_weights(or_grid* g) {
 }
 
 int or_preprocess(or_grid* g, double mach, double alpha, double cfl) {
-    set_par(&g->P, mach, alpha, cfl);
+    set_par(&g->P, mach, alpha, cfl, g->lim);
     const int nC = g->nC, nB = g->nB, n = nC + nB;
     /* SpecifyBC (load_grid.cpp:857-902) */
     g->btype = malloc(sizeof(int) * (nB + 1));
@@ -697,7 +719,7 @@ static void flux_lusgs(const double q[5], const double DQ[5], double nx, double 
 }
 
 /* DQ_limit == 2 (forward_lusgs.cpp:939-973, 1184-1218) */
-static void limiter(const or_grid* g, int cell) {
+static void limiter(or_grid* g, int cell, int fwd) {
     const Par* P = &g->P;
     const size_t n = (size_t)g->nC + g->nB;
     double* const* q = g->q;
@@ -708,8 +730,10 @@ static void limiter(const or_grid* g, int cell) {
                     (DQ[n + cell] * q[1][cell] + DQ[2 * n + cell] * q[2][cell] + DQ[3 * n + cell] * q[3][cell]);
         dp *= P->gamm1;
         if ((q[0][cell] + DQ[cell]) < P->rho_min || (q[0][cell] + DQ[cell]) > P->rho_max ||
-            (q[4][cell] + dp) < P->p_min || (q[4][cell] + dp) > P->p_max)
+            (q[4][cell] + dp) < P->p_min || (q[4][cell] + dp) > P->p_max) {
+            bump(g, (fwd ? OR_FWD_LIMIT1 : OR_BWD_LIMIT1) + stage);
             for (int i = 0; i < 5; ++i) DQ[i * n + cell] = stage < 2 ? DQ[i * n + cell] * 0.1 : 0.0;
+        }
     }
 }
 
@@ -748,7 +772,7 @@ static void sweep_cell(or_grid* g, const double* Diag, int cell, int fwd) {
         double tmp = 2.0 * Diag[cell];
         for (int i = 0; i < 5; i++) g->dq[i * n + cell] -= flux_s[i] / tmp;
     }
-    limiter(g, cell);
+    limiter(g, cell, fwd);
 }
 
 /* CompNodeVar3D_dist (cal_gradient.cpp:1922-2327) up to the division: the
@@ -851,10 +875,10 @@ static void run_iteration(or_grid* g, int threads) {
         if (p > P->p_break)
             ct = cfl;
         else if (p < P->p_min)
-            ct = g->cfl_min;
+            ct = g->cfl_min, bump(g, OR_CFL_PMIN);
         else
-            ct = (p - P->p_min) / (P->p_break - P->p_min) * (cfl - g->cfl_min) + g->cfl_min;
-        if (!det[i]) ct = g->cfl_min;
+            ct = (p - P->p_min) / (P->p_break - P->p_min) * (cfl - g->cfl_min) + g->cfl_min, bump(g, OR_CFL_RAMP);
+        if (!det[i]) ct = g->cfl_min, bump(g, OR_CFL_DET);
         dt[i] *= ct;
     }
     free(det);
@@ -865,7 +889,7 @@ static void run_iteration(or_grid* g, int threads) {
     if (g->dt_max / g->dt_min > P->ratio_dtmax) {
         double lim = P->ratio_dtmax * g->dt_min;
         for (int i = 0; i < nC; ++i)
-            if (dt[i] > lim) dt[i] = lim;
+            if (dt[i] > lim) dt[i] = lim, bump(g, OR_DTMAX);
     }
     /* ForwardLUSGS (forward_lusgs.cpp:67-163): Diag, CalDiagLUSGS (:554-595) */
     double* Diag = malloc(8 * nC);
@@ -921,14 +945,18 @@ static void run_iteration(or_grid* g, int threads) {
         v[i] = rv * rhotr;
         w[i] = rw * rhotr;
         double rho_del = g->dq[i], rho_rat = rho_del / rho[i];
-        if (rho_rat < alpq) rho_del /= betq + fabs(rho_rat) * phiq;
+        if (rho_rat < alpq) rho_del /= betq + fabs(rho_rat) * phiq, bump(g, OR_RELAX_RHO);
         rho[i] += rho_del;
+        if (rho[i] < P->rho_min) bump(g, OR_CLAMP_RHO_LO);
+        if (rho[i] > P->rho_max) bump(g, OR_CLAMP_RHO_HI);
         rho[i] = dmax(rho[i], P->rho_min);
         rho[i] = dmin(rho[i], P->rho_max);
         double ptmp = gam1 * (re - 0.5 * (u[i] * u[i] + v[i] * v[i] + w[i] * w[i]) * rho[i]);
         double p_del = ptmp - p[i], p_rat = p_del / (p[i] + P->p_bar);
-        if (p_rat < alpq) p_del /= betq + fabs(p_rat) * phiq;
+        if (p_rat < alpq) p_del /= betq + fabs(p_rat) * phiq, bump(g, OR_RELAX_P);
         p[i] += p_del;
+        if (p[i] < P->p_min) bump(g, OR_CLAMP_P_LO);
+        if (p[i] > P->p_max) bump(g, OR_CLAMP_P_HI);
         p[i] = dmax(p[i], P->p_min);
         p[i] = dmin(p[i], P->p_max);
     }
@@ -1040,9 +1068,27 @@ int or_set_ghosts(or_grid* g) {
     return 0;
 }
 
+int or_set_limits(or_grid* g, double ratio_rhop_min, double ratio_rhop_max, double ratio_p_break, double ratio_dtmax) {
+    if (!(ratio_rhop_min > 0.0) || !(ratio_rhop_max > ratio_rhop_min) || !(ratio_dtmax > 1.0))
+        return fail("or_set_limits: need 0 < ratio_rhop_min < ratio_rhop_max and ratio_dtmax > 1");
+    g->lim[0] = ratio_rhop_min;
+    g->lim[1] = ratio_rhop_max;
+    g->lim[2] = ratio_p_break;
+    g->lim[3] = ratio_dtmax;
+    if (g->res) set_par(&g->P, g->P.mach, g->P.alpha, g->P.cfl_start, g->lim); /* already preprocessed */
+    return 0;
+}
+
+int or_counts(or_grid* g, long long* dst, int n, int reset) {
+    if (n != OR_NCOUNT) return fail("or_counts: n must be OR_NCOUNT");
+    memcpy(dst, g->cnt, sizeof(g->cnt));
+    if (reset) memset(g->cnt, 0, sizeof(g->cnt));
+    return 0;
+}
+
 void or_derived(double mach, double alpha, double d[14]) {
     Par P;
-    set_par(&P, mach, alpha, 1.0);
+    set_par(&P, mach, alpha, 1.0, NULL);
     const double gam = P.gam, gamm1 = P.gamm1;
     double temp = 1.0 + 0.5 * gamm1 * mach * mach;
     double p_stag = (pow(temp, gam / gamm1) - 1.0) * P.p_bar;
@@ -1055,14 +1101,14 @@ void or_derived(double mach, double alpha, double d[14]) {
 void or_roe_flux(const double ql[5], const double qr[5], double nx, double ny, double nz, double area, double mach,
                  double alpha, double f[5]) {
     Par P;
-    set_par(&P, mach, alpha, 1.0);
+    set_par(&P, mach, alpha, 1.0, NULL);
     roe(ql, qr, nx, ny, nz, area, &P, f);
 }
 
 void or_flux_lusgs(const double q[5], const double dq[5], double nx, double ny, double nz, double mach, double alpha,
                    double f[5]) {
     Par P;
-    set_par(&P, mach, alpha, 1.0);
+    set_par(&P, mach, alpha, 1.0, NULL);
     flux_lusgs(q, dq, nx, ny, nz, &P, f);
 }
 

@@ -45,6 +45,24 @@ int or_set_ghost_refresh(or_grid* g, int on);
 int or_set(or_grid* g, const char* name, const double* src, size_t count);
 /* SetGhostVariables (load_grid.cpp:953-1212) on the current state */
 int or_set_ghosts(or_grid* g);
+/* the state limits of para_field_global.cpp:200-208 and the dt ratio limit
+ * (:115): ratio_rhop_min (default 1e-4), ratio_rhop_max (10), ratio_p_break
+ * (0.01), ratio_dtmax (1e80).  Set before or_preprocess for the set-up's
+ * ghost clamps to see them; afterwards they apply from the next iteration. */
+int or_set_limits(or_grid* g, double ratio_rhop_min, double ratio_rhop_max, double ratio_p_break, double ratio_dtmax);
+/* Guard-branch counters: how many times (cells, or cell x stage) each
+ * robustness branch was taken since or_load or the last reset. */
+enum {
+    OR_FWD_LIMIT1, OR_FWD_LIMIT2, OR_FWD_LIMIT3, /* DQ limiter stages, forward sweep (forward_lusgs.cpp:939-973) */
+    OR_BWD_LIMIT1, OR_BWD_LIMIT2, OR_BWD_LIMIT3, /* backward sweep (:1184-1218) */
+    OR_RELAX_RHO, OR_RELAX_P,                    /* CFL3D under-relaxation (:1406-1424) */
+    OR_CLAMP_RHO_LO, OR_CLAMP_RHO_HI, OR_CLAMP_P_LO, OR_CLAMP_P_HI, /* update clamps (:1413-1427) */
+    OR_CFL_RAMP, OR_CFL_PMIN, OR_CFL_DET,        /* LimitTimeStep (compute_timestep.cpp:886-902) */
+    OR_DTMAX,                                    /* ratio_dtmax clamp (:929-945) */
+    OR_GHOST_RHO_LO, OR_GHOST_RHO_HI, OR_GHOST_P_LO, OR_GHOST_P_HI, /* load_grid.cpp:1207-1210 */
+    OR_NCOUNT
+};
+int or_counts(or_grid* g, long long* dst, int n, int reset);
 void or_free(or_grid* g);
 const char* or_error(void);
 

@@ -10,9 +10,16 @@
 //
 // Nothing in the product loads or runs this program.
 //
-// Usage: ref_driver MESH BCFILE OUTDIR MACH ALPHA CFL ITERS DUMPLIST
+// Usage: ref_driver MESH BCFILE OUTDIR MACH ALPHA CFL ITERS DUMPLIST [options]
 //   DUMPLIST: comma-separated iteration counts after which the state is
 //   written (0 = after setup), e.g. "0,1,10,100".
+//   options: ghosts | nogeo | ramp=CFL_START,NSTEP |
+//            limits=RHOP_MIN,RHOP_MAX,P_BREAK (ratio_rhop_min, ratio_rhop_max,
+//            ratio_p_break of para_field_global.cpp:200-202, applied before
+//            the derived limits are computed) | dtratio=X (ratio_dtmax,
+//            :115) | init=DIR (after PreprocessGrid, rho u v w p of all
+//            nTCell+nBFace cells, ghosts included, are read from
+//            DIR/init_<name>.f64)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -120,6 +127,7 @@ int main(int argc, char** argv) {
     bool dump_geometry = true;  // "nogeo": skip the set-up geometry dumps (large meshes)
     double ramp_start = 0.0;
     int ramp_nstep = 0;
+    std::string init_dir;
     for (int a = 9; a < argc; ++a) {
         const std::string s = argv[a];
         if (s == "ghosts") ghost_refresh = true;
@@ -128,6 +136,14 @@ int main(int argc, char** argv) {
             ramp_start = atof(s.substr(5).c_str());
             ramp_nstep = atoi(s.substr(s.find(',') + 1).c_str());
         }
+        if (s.rfind("limits=", 0) == 0) {
+            if (sscanf(s.c_str() + 7, "%lf,%lf,%lf", &ratio_rhop_min, &ratio_rhop_max, &ratio_p_break) != 3) {
+                fprintf(stderr, "bad %s\n", s.c_str());
+                return 2;
+            }
+        }
+        if (s.rfind("dtratio=", 0) == 0) ratio_dtmax = atof(s.c_str() + 8);
+        if (s.rfind("init=", 0) == 0) init_dir = s.substr(5);
     }
     const std::string mesh = argv[1], bcf = argv[2], out = argv[3];
     const double m_in = atof(argv[4]), a_in = atof(argv[5]), cfl = atof(argv[6]);
@@ -157,6 +173,20 @@ int main(int argc, char** argv) {
     ReadMFlowGrid_binary(grid, mesh);
     ReadMFlowBoundaryCondition(bc, bcf.c_str());
     PreprocessGrid(grid, bc);
+    if (!init_dir.empty()) {
+        const size_t n = (size_t)grid->GetNTCell() + grid->GetNBFace();
+        RealFlow* dst[5] = {grid->GetRho(), grid->GetU(), grid->GetV(), grid->GetW(), grid->GetP()};
+        const char* nm[5] = {"rho", "u", "v", "w", "p"};
+        for (int j = 0; j < 5; ++j) {
+            const std::string path = init_dir + "/init_" + nm[j] + ".f64";
+            FILE* f = fopen(path.c_str(), "rb");
+            if (!f || fread(dst[j], sizeof(double), n, f) != n) {
+                fprintf(stderr, "cannot read %zu doubles from %s\n", n, path.c_str());
+                return 2;
+            }
+            fclose(f);
+        }
+    }
 
     const int nC = grid->GetNTCell(), nB = grid->GetNBFace(), nF = grid->GetNTFace(), nN = grid->GetNTNode();
     if (dump_geometry) {

@@ -100,6 +100,52 @@ def test_contexts_are_independent(tmp_path):
     b.close()
 
 
+def test_threads_stepping_contexts_on_one_device():
+    """Two host threads step a context each, and a third steps a batch, at
+    the same time on one device.  Each sweep grid needs all of its waves
+    resident, so the library orders the sweeps of different streams on a
+    device one after the other (the other kernels may overlap).  Every
+    result equals the same solve stepped alone."""
+    import threading
+
+    print("[api] three threads stepping on one device", flush=True)
+    cases = [(mfs.make_params(mach=m, alpha=a, cfl=10.0), n) for m, a, n in ((0.7, 1.0, 14), (0.5, -2.0, 12), (0.8, 3.0, 10))]
+    grids = [mfs.Grid.box_bump(n, seed=31 + n).decouple().preprocess(p) for p, n in cases]
+    iters = 40
+    ref = []
+    for g, (p, _) in zip(grids, cases):
+        s = mfs.Solver(g, p)
+        s.step(iters)
+        ref.append(({nm: s.field(nm) for nm in FIELDS}, s.history()))
+        s.close()
+    solvers = [mfs.Solver(grids[0], cases[0][0]), mfs.Solver(grids[1], cases[1][0])]
+    batch = mfs.Batch(mfs.MFS_FP64)
+    member = batch.add(grids[2], cases[2][0])
+    errors = []
+
+    def run(stepper):
+        try:
+            for _ in range(iters // 5):
+                stepper.step(5)
+            stepper.sync()
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=run, args=(x,)) for x in (solvers[0], solvers[1], batch)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for s, (fields, hist) in zip(solvers + [member], ref):
+        for nm in FIELDS:
+            np.testing.assert_array_equal(s.field(nm), fields[nm], err_msg=nm)
+        np.testing.assert_array_equal(s.history(), hist)
+    for s in solvers:
+        s.close()
+    batch.close()
+
+
 @pytest.mark.parametrize("tile_cells,sweep_waves,by_tiles", [(64, 4, 0), (1000, 16, 1), (8192, 8, 0), (64, 8, 1)])
 def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles):
     """Tile size, sweep workgroup size and sweep schedule change the device

@@ -64,16 +64,17 @@ def test_mixed_converges_to_the_fp64_solution(mach, alpha):
 @pytest.mark.parametrize("waves", [4, 8, 16])
 def test_full_sweep_grid_is_resident(precision, waves):
     """A mesh with more sweep chunks than the sweep grid has waves, so every
-    workgroup of the cooperative grid is launched and must be resident for
-    the sweeps to finish: at most 16 waves per CU, for every workgroup size
-    and both operator precisions.  The mixed run's first residual is the
-    FP64 one."""
+    workgroup of the sweep grid is launched and the sweeps need them all
+    resident at once: at most 16 waves per CU (a measured bound, not an
+    occupancy query), for every workgroup size and both operator
+    precisions."""
     print(f"[mixed] full sweep grid precision={precision} waves={waves}", flush=True)
     p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
     g = mfs.Grid.box_bump(25).decouple().preprocess(p)
     s = mfs.Solver(g, p, precision=precision, sweep_waves=waves)
     info = s.info()
-    assert info["sweep_workgroups"] * waves <= 16 * 256
+    assert info["compute_units"] > 0
+    assert info["sweep_workgroups"] * waves <= 16 * info["compute_units"]
     assert g.info["nTCell"] // 16 > info["sweep_workgroups"] * waves  # more chunks than waves
     s.step(3)
     h = s.history()

@@ -154,29 +154,6 @@ def test_tiny_and_ragged_boxes_bitwise_vs_reference(tmp_path, ref_driver, dims):
         s.close()
 
 
-def test_full_size_bitwise_vs_reference(tmp_path, ref_driver):
-    """The 100^3 x 6 = 6.0M-cell box+bump mesh (the throughput workload),
-    two iterations: state, residual, time step, DQ and gradients bitwise
-    against the reference's serial build on the same mesh."""
-    print("[parity] 6.0M cells, 2 iterations (the reference build takes about a minute)", flush=True)
-    mach, alpha, cfl, iters = 0.5, 0.0, 10.0, 2
-    g = mfs.Grid.box_bump(100).decouple()
-    mesh, bc = str(tmp_path / "m.mfl"), str(tmp_path / "m.bc")
-    g.save(mesh, bc)
-    out = str(tmp_path / "ref")
-    ref = run_reference(ref_driver, mesh, bc, out, mach, alpha, cfl, iters, dumps=(iters,), geometry=False)
-    p = mfs.make_params(mach=mach, alpha=alpha, cfl=cfl)
-    g.preprocess(p, device=True)
-    s = mfs.Solver(g, p)
-    s.step(iters)
-    _assert_state_equal(s, out, iters, g.info["nTCell"])
-    h = s.history()
-    np.testing.assert_array_equal(h[:, 5:], ref["history"][:, 5:])
-    np.testing.assert_allclose(h[:, :5], ref["history"][:, :5], rtol=HIST_RTOL, atol=0)
-    s.close()
-    g.close()
-
-
 @pytest.mark.parametrize("cells", ["hex", "prism"])
 def test_polyhedral_cells_bitwise_vs_reference(tmp_path, ref_driver, cells):
     """Hexahedra (6 quadrilateral faces) and prisms (5 faces, mixed
@@ -241,4 +218,67 @@ def test_ghost_refresh_vs_reference(tmp_path, ref_driver):
     print(f"[parity] ghost refresh: fraction of bitwise-equal state entries {min(exact):.6f}", flush=True)
     np.testing.assert_allclose(s.history(), ref["history"], rtol=1e-10, atol=0)
     s.close()
+    g.close()
+
+
+@pytest.mark.parametrize("ghost_refresh", [False, True], ids=["plain", "ghost-refresh"])
+def test_guard_branches_vs_reference(tmp_path, ref_driver, ghost_refresh):
+    """The robustness branches (all three DQ limiter stages in both sweeps,
+    CFL3D relaxation, the update's rho/p clamps, LimitTimeStep below p_break
+    and p_min, CellIsMG, the ratio_dtmax clamp, and with the ghost refresh
+    the ghost clamps) on a perturbed state with tightened limits
+    (tests/guards.py), uploaded with mfs_upload, against the reference build
+    started from the same state.  Plain: bitwise after each of three
+    iterations.  With the ghost refresh the far-field ghosts go through pow,
+    whose GPU result may differ in the last bit, so the state is compared to
+    1e-12 of each field's scale.  The oracle is pinned on the same workload
+    (tests/test_oracle.py); here each branch's fingerprint is counted in the
+    GPU's own output."""
+    from tests import guards
+
+    print(f"[parity] guard branches ghost_refresh={ghost_refresh}", flush=True)
+    mesh, bc, info, st, init = guards.make_case(tmp_path)
+    nC = info["nTCell"]
+    out = str(tmp_path / "ref")
+    hist_r = guards.run_reference(ref_driver, mesh, bc, init, out, ghost_refresh)
+    p = guards.params()
+    g = mfs.Grid.load(mesh, bc).preprocess(p)
+    s = mfs.Solver(g, p)
+    for k, a in st.items():
+        s.upload(k, a)
+    s.ghost_refresh(ghost_refresh)
+    gpu_dir = tmp_path / "gpu"
+    os.makedirs(gpu_dir)
+    for it in range(1, guards.ITERS + 1):
+        s.step(1)
+        if not ghost_refresh:
+            _assert_state_equal(s, out, it, nC)
+        else:
+            for nm in STATE:
+                a, b = s.field(nm), load_state(out, it, nm)
+                scale = np.max(np.abs(b)) if nm != "p" else p.p_bar
+                np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * scale, err_msg=f"it{it} {nm}")
+        for nm in ("rho", "p", "dt", "dq", "u", "v", "w"):
+            s.field(nm).tofile(str(gpu_dir / f"it{it}_{nm}.f64"))
+    h = s.history()
+    if not ghost_refresh:
+        np.testing.assert_array_equal(h[:, 5:], hist_r[:, 5:])
+        np.testing.assert_allclose(h[:, :5], hist_r[:, :5], rtol=HIST_RTOL, atol=0)
+    else:
+        np.testing.assert_allclose(h, hist_r, rtol=1e-10, atol=0)
+    ev = guards.branch_predicates(str(gpu_dir), st, h, nC, ghost_refresh)
+    print(f"[parity] guard fingerprints in the GPU output: {ev}", flush=True)
+    assert all(v > 0 for v in ev.values()), ev
+    s.close()
+    if not ghost_refresh:  # the FP32 operator takes the same branches and stays finite
+        m = mfs.Solver(g, p, precision=mfs.MFS_MIXED)
+        for k, a in st.items():
+            m.upload(k, a)
+        m.step(guards.ITERS)
+        rho, pr = m.field("rho"), m.field("p")
+        assert np.isfinite(m.history()).all() and np.isfinite(rho).all() and np.isfinite(pr).all()
+        for a, lo, hi in ((rho, p.rho_min, p.rho_max), (pr, p.p_min, p.p_max)):
+            assert np.all((a[:nC] >= lo) & (a[:nC] <= hi))
+            assert np.any(a[:nC] == lo) and np.any(a[:nC] == hi)
+        m.close()
     g.close()

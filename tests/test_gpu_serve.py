@@ -22,7 +22,7 @@ def test_served_results_equal_alone(precision):
     run_sequential(alone, drop=1e-6, max_iters=4000, check_every=20, precision=precision)
     for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(alone, key=lambda r: r.rid)):
         assert a.iters == b.iters, (a.rid, a.iters, b.iters)
-        assert a.rho_sum == b.rho_sum, (a.rid, a.rho_sum, b.rho_sum)
+        assert a.state_sha256 == b.state_sha256, (a.rid, a.state_sha256, b.state_sha256)
         assert a.latency >= 0 and np.isfinite(a.latency)
 
 

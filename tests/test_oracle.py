@@ -356,3 +356,44 @@ def test_freestream_residual_vanishes_away_from_the_wall(tmp_path):
     scale = p_bar * (1.0 / 6) ** 2
     assert np.abs(res[:, mask]).max() < 1e-9 * scale
     assert np.abs(res[:, ~mask]).max() > 1e-6 * scale  # the bump deflects the flow
+
+
+# every guard branch the oracle counts; the ghost clamps need the ghost refresh
+GUARD_BRANCHES = ("fwd_limit1", "fwd_limit2", "fwd_limit3", "bwd_limit1", "bwd_limit2", "bwd_limit3",
+                  "relax_rho", "relax_p", "clamp_rho_lo", "clamp_rho_hi", "clamp_p_lo", "clamp_p_hi",
+                  "cfl_ramp", "cfl_pmin", "cfl_det", "dtmax")
+GHOST_BRANCHES = ("ghost_rho_lo", "ghost_rho_hi", "ghost_p_lo", "ghost_p_hi")
+
+
+@pytest.mark.parametrize("ghost_refresh", [False, True], ids=["plain", "ghost-refresh"])
+@pytest.mark.parametrize("threads", [0, 4])
+def test_guard_branches_oracle_vs_reference_build(tmp_path, ref_driver, ghost_refresh, threads):
+    """The robustness branches (DQ limiter stages, CFL3D relaxation, update
+    and ghost clamps, the p_break / p_min CFL, CellIsMG, the ratio_dtmax
+    clamp) on a perturbed state with tightened limits: the oracle equals the
+    reference build bit for bit after each of three iterations, the oracle
+    counts every branch as taken, and the reference's own dumps show the
+    branches' fingerprints (values exactly on a bound, zeroed DQ, dt cut to
+    ratio_dtmax times the minimum)."""
+    from tests import guards
+
+    mesh, bc, info, st, init = guards.make_case(tmp_path)
+    out = str(tmp_path / "ref")
+    hist_r = guards.run_reference(ref_driver, mesh, bc, init, out, ghost_refresh)
+    o = Oracle(mesh, bc, guards.MACH, guards.ALPHA, guards.CFL, limits=guards.LIMITS)
+    o.counts(reset=True)
+    for k, a in st.items():
+        o.set(k, a)
+    o.set_ghost_refresh(ghost_refresh)
+    hist_o = []
+    for it in range(1, guards.ITERS + 1):
+        h, _ = o.run(1, threads)
+        hist_o.append(h)
+        for nm in FIELDS:
+            np.testing.assert_array_equal(o.get(nm), guards.load(out, it, nm), err_msg=f"it{it} {nm}")
+    np.testing.assert_array_equal(np.vstack(hist_o), hist_r)
+    counts = o.counts()
+    want = GUARD_BRANCHES + (GHOST_BRANCHES if ghost_refresh else ())
+    assert all(counts[k] > 0 for k in want), {k: counts[k] for k in want}
+    ev = guards.branch_predicates(out, st, hist_r, info["nTCell"], ghost_refresh)
+    assert all(v > 0 for v in ev.values()), ev
