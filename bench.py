@@ -10,8 +10,9 @@ The mesh and state are resident in HBM before the timed region.
 
 Prints one JSON line.  `roofline` describes the kernel with the largest
 share of the timed region, timed with HIP events on the solver stream;
-`cpu_baseline` times the reference's own serial build (oracle/_ref) on a
-bounded sample of the same workload on this host; `mixed` repeats the
+`cpu_baseline` times the reference's own OpenMP build (oracle/_ref) on all
+host threads for a few iterations of the same mesh and flow, with its
+serial build beside it; `mixed` repeats the
 timed region with the mixed-precision context (FP32 implicit operator,
 FP64 residual and update) on the same mesh.
 
@@ -78,17 +79,6 @@ def alg_bytes(coef, nF, nC, nB, nN):
     return coef[0] * nF + coef[1] * nC + coef[2] * nB + coef[3] * nN
 
 
-def _cpu_mesh(td: str, grid_n: int, seed: int):
-    import minifs_amd as mfs
-
-    g = mfs.Grid.box_bump(grid_n, seed=seed).decouple()
-    mesh, bc = os.path.join(td, "m.mfl"), os.path.join(td, "m.bc")
-    g.save(mesh, bc)
-    nC = g.info["nTCell"]
-    g.close()
-    return mesh, bc, nC
-
-
 def host_cpu() -> dict:
     """The host's CPU model, logical CPU count and the OpenMP binding in use."""
     model = None
@@ -103,59 +93,64 @@ def host_cpu() -> dict:
     return {"cpu_model": model, "nproc": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
 
-def cpu_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: float, seed: int) -> dict | None:
-    """Times the reference's own serial build (oracle/_ref/ref_driver, built
-    from the reference sources with g++ -O3) on a bounded sample of the same
-    workload: `iters` iterations of the n^3 x 6 box+bump mesh, loop time only."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    if not os.path.exists(ref):
-        return None
-    with tempfile.TemporaryDirectory() as td:
-        mesh, bc, nC = _cpu_mesh(td, grid_n, seed)
-        out = os.path.join(td, "ref")
-        os.makedirs(out)
-        subprocess.run([ref, mesh, bc, out, repr(mach), repr(alpha), repr(cfl), str(iters), ""], check=True,
-                       stdout=subprocess.DEVNULL, timeout=1200)
+def _run_reference(exe: str, mesh: str, bc: str, iters: int, mach: float, alpha: float, cfl: float,
+                   env: dict | None = None) -> tuple[float, float]:
+    """Runs a reference build (oracle/_ref) for `iters` iterations on a saved
+    mesh; returns (loop time, whole-run wall time) in seconds."""
+    with tempfile.TemporaryDirectory() as out:
+        t0 = time.perf_counter()
+        subprocess.run([exe, mesh, bc, out, repr(mach), repr(alpha), repr(cfl), str(iters), "", "nogeo"], check=True,
+                       stdout=subprocess.DEVNULL, timeout=1200, env=env)
+        wall = time.perf_counter() - t0
         with open(os.path.join(out, "summary.txt")) as f:
             first = f.readline().split()
-        kv = dict(zip(first[0::2], first[1::2]))
-        t_loop = float(kv["t_loop"])
-        return {
-            "value": round(nC * iters / t_loop / 1e6, 4),
+    kv = dict(zip(first[0::2], first[1::2]))
+    return float(kv["t_loop"]), wall
+
+
+def cpu_samples(mesh: str, bc: str, nC: int, label: str, args) -> tuple[dict | None, dict | None]:
+    """The CPU path beside the GPU, on the same saved mesh.
+
+    cpu_baseline: the reference's own OpenMP face-colouring build (command:22:
+    -fopenmp -DMF_OPENMP -DOMP_CellColor -DOMP_FaceColor, built from the
+    reference sources into oracle/_ref/ref_driver_omp) on all host threads
+    with OMP_PROC_BIND=close, `cpu_iters` iterations, loop time only
+    (src/main.cpp:183-196, 353-354).  Its LU-SGS runs over cell colours, so
+    its sums differ from the serial order: a timing baseline, not a parity
+    reference.
+    The serial reference build (oracle/_ref/ref_driver) is timed beside it on
+    the same mesh for `cpu_serial_iters` iterations."""
+    omp = os.path.join(ROOT, "oracle", "_ref", "ref_driver_omp")
+    ser = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    base = serial = None
+    if os.path.exists(omp) and args.cpu_iters > 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        t_loop, wall = _run_reference(omp, mesh, bc, args.cpu_iters, args.mach, args.alpha, args.cfl, env)
+        host = host_cpu()
+        host["omp_proc_bind"] = "close"
+        base = {
+            "value": round(nC * args.cpu_iters / t_loop / 1e6, 4),
+            "unit": "M cell-iters/s",
+            "cores": threads,
+            "kind": "reference",
+            "sample": f"reference OpenMP face-colouring build (oracle/_ref/ref_driver_omp: g++ -O3 -fopenmp "
+                      f"-DMF_OPENMP -DOMP_CellColor -DOMP_FaceColor), {threads} threads, OMP_PROC_BIND=close, "
+                      f"{args.cpu_iters} iterations of the {label} = {nC} cell mesh, same flow condition, loop "
+                      f"time only ({t_loop:.1f} s of {wall:.1f} s incl. mesh read and set-up)",
+            "host": host,
+        }
+    if os.path.exists(ser) and args.cpu_serial_iters > 0:
+        t_loop, wall = _run_reference(ser, mesh, bc, args.cpu_serial_iters, args.mach, args.alpha, args.cfl)
+        serial = {
+            "value": round(nC * args.cpu_serial_iters / t_loop / 1e6, 4),
             "unit": "M cell-iters/s",
             "cores": 1,
             "kind": "reference",
-            "sample": f"reference serial build (oracle/_ref/ref_driver, g++ -O3 -ffp-contract=off), {iters} "
-                      f"iterations of the {grid_n}^3x6 = {nC} cell box+bump mesh, same flow condition, "
-                      f"loop time only ({t_loop:.1f} s)",
-            "host": host_cpu(),
+            "sample": f"reference serial build (oracle/_ref/ref_driver, g++ -O3 -ffp-contract=off), "
+                      f"{args.cpu_serial_iters} iterations of the same mesh, loop time only ({t_loop:.1f} s)",
         }
-
-
-def cpu_port_sample(grid_n: int, iters: int, mach: float, alpha: float, cfl: float, seed: int) -> dict | None:
-    """Times the C restatement (oracle/liboracle.so) with OpenMP on all host
-    threads: per-cell gathers and level-scheduled sweeps, the arithmetic of
-    the reference's serial loop."""
-    from oracle import LIB, Oracle
-
-    if not os.path.exists(LIB):
-        return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    os.environ.setdefault("OMP_PROC_BIND", "close")  # read by the OpenMP runtime when the library starts it
-    with tempfile.TemporaryDirectory() as td:
-        mesh, bc, nC = _cpu_mesh(td, grid_n, seed)
-        o = Oracle(mesh, bc, mach, alpha, cfl)
-        _, t_loop = o.run(iters, threads)
-        o.close()
-    return {
-        "value": round(nC * iters / t_loop / 1e6, 4),
-        "unit": "M cell-iters/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"C restatement, {threads} OpenMP threads, {iters} iterations of the {grid_n}^3x6 = {nC} cell "
-                  f"box+bump mesh, loop time only ({t_loop:.1f} s)",
-        "host": host_cpu(),
-    }
+    return base, serial
 
 
 def copy_calibration(mib: int, reps: int) -> int:
@@ -196,12 +191,15 @@ def timed_steps(s, steps: int, warmup: int, kernel_timing: bool):
     return elapsed, kern
 
 
-def mode_fp64(args) -> dict:
+def mode_fp64(args, cpu_dir: str | None = None) -> dict:
     import minifs_amd as mfs
 
     t0 = time.time()
     p = mfs.make_params(mach=args.mach, alpha=args.alpha, cfl=args.cfl)
-    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple(*args.tree).preprocess(p)
+    g = mfs.Grid.box_bump(args.n, seed=args.seed).decouple(*args.tree)
+    if cpu_dir:  # the CPU path reads the same mesh (the reference's FlowStar format)
+        g.save(os.path.join(cpu_dir, "m.mfl"), os.path.join(cpu_dir, "m.bc"))
+    g.preprocess(p)
     info = g.info
     s = mfs.Solver(g, p)
     setup_s = time.time() - t0
@@ -280,7 +278,7 @@ def mode_fp64(args) -> dict:
         "setup_s": round(setup_s, 2),
         "last_residual": [float(x) for x in hist[-1][:5]] if len(hist) else None,
         "cpu_baseline": None,
-        "cpu_port_threaded": None,
+        "cpu_reference_serial": None,
     }
 
 
@@ -394,8 +392,10 @@ def mode_batch(args) -> dict:
     with cf.ThreadPoolExecutor(min(16, os.cpu_count() or 4)) as pool:
         built = list(pool.map(build, specs))
     setup_s = time.time() - t0
+    from minifs_amd.serve import tuned_solver
+
     b = mfs.Batch(prec)
-    members = [b.add(g, p) for g, p in built]
+    members = [b.attach(tuned_solver(g, p, prec)) for g, p in built]
     cells = sum(g.info["nTCell"] for g, _ in built)
     b.step(args.warmup)
     b.sync()
@@ -411,7 +411,7 @@ def mode_batch(args) -> dict:
     if not args.no_sequential:
         t2 = time.perf_counter()
         for g, p in built:
-            s = mfs.Solver(g, p, precision=prec)
+            s = tuned_solver(g, p, prec)
             s.step(args.steps)
             s.sync()
             s.close()
@@ -465,9 +465,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--tree", type=lambda t: tuple(int(x) for x in t.split(",")), default=(4, 512),
                     metavar="PARTS,LEAF", help="TDTree parameters of the mesh order (the reference's: 4,512)")
-    ap.add_argument("--cpu-n", type=int, default=40, help="mesh of the CPU baseline sample (0 = skip)")
-    ap.add_argument("--cpu-iters", type=int, default=20)
-    ap.add_argument("--cpu-port-iters", type=int, default=60)
+    ap.add_argument("--cpu-iters", type=int, default=10,
+                    help="iterations of the reference's OpenMP build on the bench mesh (0 = skip)")
+    ap.add_argument("--cpu-serial-iters", type=int, default=2,
+                    help="iterations of the reference's serial build on the bench mesh (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision repeat of the timed region")
     ap.add_argument("--no-shallow", action="store_true", help="skip the repeat on the shallower cell order")
@@ -512,18 +513,16 @@ def main() -> int:
     elif args.mode == "serve":
         line = mode_serve(args)
     else:
-        line = mode_fp64(args)
-        if rank == 0 and args.cpu_n > 0:
-            try:
-                line["cpu_baseline"] = cpu_sample(args.cpu_n, args.cpu_iters, args.mach, args.alpha, args.cfl,
-                                                  args.seed)
-            except Exception as e:  # reported, not fatal: the GPU number stands on its own
-                line["cpu_baseline"] = {"error": str(e)}
-            try:
-                line["cpu_port_threaded"] = cpu_port_sample(args.cpu_n, args.cpu_port_iters, args.mach, args.alpha,
-                                                            args.cfl, args.seed)
-            except Exception as e:
-                line["cpu_port_threaded"] = {"error": str(e)}
+        cpu = rank == 0 and (args.cpu_iters > 0 or args.cpu_serial_iters > 0)
+        with tempfile.TemporaryDirectory() as td:
+            line = mode_fp64(args, td if cpu else None)
+            if cpu:
+                try:
+                    line["cpu_baseline"], line["cpu_reference_serial"] = cpu_samples(
+                        os.path.join(td, "m.mfl"), os.path.join(td, "m.bc"), line["config"]["cells"],
+                        f"{args.n}^3x6 box+bump", args)
+                except Exception as e:  # reported, not fatal: the GPU number stands on its own
+                    line["cpu_baseline"] = {"error": str(e)}
     if rank == 0:
         print(json.dumps(line), flush=True)
     return 0

@@ -167,6 +167,10 @@ typedef struct mfs_ctx_options {
     int tile_cells;     /* cells per storage tile of the LU-SGS sweeps (>= 64; default 4096) */
     int sweep_waves;    /* waves per sweep workgroup: 4, 8 or 16 (default 4) */
     int sweep_by_tiles; /* 0: flat level-order schedule (default); 1: one workgroup per tile */
+    int gradient_stream; /* mfs_step's CalculateGradient: 1 on the context stream, fused with the next
+                          * residual pass (default); 2 on a second stream of the context, beside the
+                          * next iteration's residual pass and sweeps (nothing in an iteration reads
+                          * the gradient) */
 } mfs_ctx_options;
 int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p, int precision, const mfs_ctx_options* opt,
                       mfs_ctx** out);

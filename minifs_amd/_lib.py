@@ -63,7 +63,8 @@ class MfsCtxInfo(C.Structure):
 
 
 class MfsCtxOptions(C.Structure):
-    _fields_ = [("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_by_tiles", C.c_int)]
+    _fields_ = [("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_by_tiles", C.c_int),
+                ("gradient_stream", C.c_int)]
 
 
 # Every symbol declared in include/mfs.h, with its ctypes signature.

@@ -92,6 +92,16 @@ struct mfs_ctx {
     double* sdn = nullptr;   // per slot: |(x_fc - x_cc).n| on this cell's side
     int* sflip = nullptr;    // per cell: slots whose face has this cell second (bits 0-7), face count (8-11)
     int ghost_refresh = 0;   // SetGhostVariables after each update in mfs_step
+    // mfs_step's gradient on a second stream (gradient_stream option 2):
+    // iteration k's node values and gradient run beside iteration k+1's
+    // residual pass and sweeps.  sdone: recorded on `stream` after each
+    // update; gdone[k & 1]: on `gstream` after iteration k's gradient, which
+    // reads the state buffer that the backward sweep of iteration k+2
+    // overwrites; gpend[j]: gdone[j] recorded and not yet waited on.
+    hipStream_t gstream = nullptr;
+    hipEvent_t sdone = nullptr, gdone[2] = {nullptr, nullptr};
+    bool gpend[2] = {false, false};
+    int gradient_stream = 1;  // 1: on the solver stream (fused with the next residual pass), 2: second stream
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;
     int nlev = 0;

@@ -63,6 +63,10 @@ constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wa
 // sweep of 6M cells); 64 counters cost well under a tenth of a sweep.
 constexpr int kTicketGroups = 64;
 constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
+// Where mfs_step runs CalculateGradient: 1 on the solver stream, fused with
+// the next iteration's residual pass; 2 on a second stream, beside the next
+// iteration's residual pass and sweeps.
+constexpr int kDefaultGradientStream = 1;
 
 #include "mfs_kernels.hpp"
 #include "mfs_sweep.hpp"
@@ -248,16 +252,17 @@ inline unsigned nblk_xcd(size_t n) { return (nblk(n) + dev::kXcd - 1) / dev::kXc
 struct Timed {
     mfs_ctx* ctx;
     const char* name;
+    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(mfs_ctx* c, const char* nm) : ctx(c), name(nm) {
+    Timed(mfs_ctx* c, const char* nm, hipStream_t s = nullptr) : ctx(c), name(nm), st(s ? s : c->stream) {
         if (!ctx->timing) return;
         a = take();
         b = take();
-        if (a) (void)hipEventRecord(a, ctx->stream);
+        if (a) (void)hipEventRecord(a, st);
     }
     ~Timed() {
         if (!ctx->timing || !a || !b) return;
-        (void)hipEventRecord(b, ctx->stream);
+        (void)hipEventRecord(b, st);
         ctx->pending.push_back({name, {a, b}});
     }
     hipEvent_t take() {
@@ -308,6 +313,25 @@ struct SweepOrder {
 };
 SweepOrder* const g_sweep_order = new SweepOrder[kMaxDevices];
 
+// Creates the device's ordering event, once; called with `device` current
+// when a context or a batch is created there, so that stepping never
+// creates one.
+bool sweep_order_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device id out of range for the sweep order table");
+        return false;
+    }
+    SweepOrder& o = g_sweep_order[device];
+    std::lock_guard<std::mutex> lk(o.mu);
+    if (o.last) return true;
+    if (hipEventCreateWithFlags(&o.last, hipEventDisableTiming) != hipSuccess) {
+        o.last = nullptr;
+        set_error("hipEventCreate(sweep order)");
+        return false;
+    }
+    return true;
+}
+
 // Holds the device's lock from before the sweeps are enqueued until the
 // event is recorded after them.
 class SweepTurn {
@@ -317,9 +341,8 @@ class SweepTurn {
             err_ = "device id out of range for the sweep order table";
             return;
         }
-        if (!o_.last && hipEventCreateWithFlags(&o_.last, hipEventDisableTiming) != hipSuccess) {
-            o_.last = nullptr;
-            err_ = "hipEventCreate(sweep order)";
+        if (!o_.last) {
+            err_ = "sweep order event of the device not created";
             return;
         }
         if (o_.last_stream && o_.last_stream != st_ && hipStreamWaitEvent(st_, o_.last, 0) != hipSuccess)
@@ -541,7 +564,8 @@ bool launch_sweeps(mfs_ctx* ctx) {
 // cfl of the next iteration in next_cfl), a tetrahedral context launches
 // the gradient fused with the next iteration's residual / time-step pass.
 template <int MAXF, typename R>
-bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0) {
+bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, hipStream_t st = nullptr) {
+    if (!st) st = ctx->stream;
     const int n = ctx->nC + ctx->nB;
     const double* q = ctx->q[ctx->cur];
     constexpr bool F64 = sizeof(R) == sizeof(double);
@@ -565,43 +589,43 @@ bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0) {
         gz = ctx->gz32;
     }
     {
-        Timed t(ctx, "node");
-        hipLaunchKernelGGL(dev::k_node<R>, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, ctx->stream, ctx->nN, ctx->nlo,
+        Timed t(ctx, "node", st);
+        hipLaunchKernelGGL(dev::k_node<R>, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, st, ctx->nN, ctx->nlo,
                            ctx->ell_off, ctx->ell_ab, ew, ctx->nden, q, qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
-            hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, ctx->stream, ctx->nsym,
+            hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, st, ctx->nsym,
                                ctx->sym_node, ctx->sym_n, qn);
             if (!check_launch(ctx, "k_node_symm")) return false;
         }
     }
     if (MAXF == 4 && ctx->snode && next) {
-        Timed t(ctx, "grad_flux_ts");
+        Timed t(ctx, "grad_flux_ts", st);
         const typename dev::Real<R>::vec4* sgeo_r;
         if constexpr (F64)
             sgeo_r = ctx->sgeo;
         else
             sgeo_r = ctx->sgeo32;
         constexpr bool MIXED = !F64;
-        hipLaunchKernelGGL((dev::k_grad_flux_ts<MIXED, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL((dev::k_grad_flux_ts<MIXED, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, st,
                            ctx->nC, n, ctx->snb, sgeo_r, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz,
                            ctx->sgeo, ctx->sdn, ctx->sflip, ctx->res, ctx->dt, ctx->diag, ctx->part, ctx->K, next_cfl,
                            0.5 * next_cfl, MIXED ? ctx->q32 : nullptr, MIXED ? ctx->diag32 : nullptr);
         return check_launch(ctx, "k_grad_flux_ts");
     }
     {
-        Timed t(ctx, "gradient");
+        Timed t(ctx, "gradient", st);
         if (MAXF == 4 && ctx->snode) {
             const typename dev::Real<R>::vec4* sgeo;
             if constexpr (F64)
                 sgeo = ctx->sgeo;
             else
                 sgeo = ctx->sgeo32;
-            hipLaunchKernelGGL(dev::k_grad_tet<R>, dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, n,
+            hipLaunchKernelGGL(dev::k_grad_tet<R>, dim3(ctx->nblk_cell), dim3(kBlock), 0, st, ctx->nC, n,
                                ctx->snb, sgeo, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz);
             if (!check_launch(ctx, "k_grad_tet")) return false;
         } else {
-            hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC,
+            hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, st, ctx->nC,
                                n, ctx->cfo, ctx->cf, ctx->fcell, fgeo, ctx->fno, ctx->fn, ctx->btype, q, qn, ctx->vol,
                                gx, gy, gz);
             if (!check_launch(ctx, "k_grad")) return false;
@@ -622,12 +646,12 @@ bool launch_ghosts(mfs_ctx* ctx) {
     return check_launch(ctx, "k_ghost");
 }
 
-bool launch_gradient(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0) {
+bool launch_gradient(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, hipStream_t st = nullptr) {
     if (ctx->precision == MFS_MIXED)
-        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx, next, next_cfl)
-                              : launch_gradient_t<6, float>(ctx, next, next_cfl);
-    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx, next, next_cfl)
-                          : launch_gradient_t<6, double>(ctx, next, next_cfl);
+        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx, next, next_cfl, st)
+                              : launch_gradient_t<6, float>(ctx, next, next_cfl, st);
+    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx, next, next_cfl, st)
+                          : launch_gradient_t<6, double>(ctx, next, next_cfl, st);
 }
 // whether launch_gradient(ctx, true, ...) also runs the next residual pass
 bool gradient_fuses_next(const mfs_ctx* ctx) { return ctx->maxf == 4 && ctx->snode; }
@@ -666,6 +690,13 @@ void free_all(mfs_ctx* ctx) {
         (void)hipStreamDestroy(ctx->stream);
     }
     ctx->stream = nullptr;
+    if (ctx->gstream) (void)hipStreamDestroy(ctx->gstream);
+    ctx->gstream = nullptr;
+    for (hipEvent_t* e : {&ctx->sdone, &ctx->gdone[0], &ctx->gdone[1]})
+        if (*e) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
 }
 
 template <int LPC, typename R>
@@ -698,8 +729,13 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     const int tile_cells = (opt && opt->tile_cells) ? opt->tile_cells : kDefaultTileCells;
     const int sweep_waves = (opt && opt->sweep_waves) ? opt->sweep_waves : 4;
     const int sweep_by_tiles = (opt && opt->sweep_by_tiles) ? 1 : 0;
+    const int gradient_stream = (opt && opt->gradient_stream) ? opt->gradient_stream : dev::kDefaultGradientStream;
     if (tile_cells < 64 || (sweep_waves != 4 && sweep_waves != 8 && sweep_waves != 16)) {
         set_error("mfs_ctx_create: tile_cells must be >= 64 and sweep_waves 4, 8 or 16");
+        return MFS_ERR_ARG;
+    }
+    if (gradient_stream != 1 && gradient_stream != 2) {
+        set_error("mfs_ctx_create: gradient_stream must be 0 (default), 1 or 2");
         return MFS_ERR_ARG;
     }
     *out = nullptr;
@@ -752,11 +788,17 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     ctx->ntiles = (int)L.tile_chunk.size() - 1;
     ctx->lpc = L.lpc;
     ctx->nblk_cell = (int)nblk_xcd(L.nC);
-    ctx->node_of_dev = L.node_of_dev;
-    ctx->cell_of_dev = L.cell_of_dev;
+    ctx->node_of_dev.assign(L.node_of_dev.begin(), L.node_of_dev.end());
+    ctx->cell_of_dev.assign(L.cell_of_dev.begin(), L.cell_of_dev.end());
     ctx->cell0 = L.dev_cell0;
     const size_t n = (size_t)L.nC + L.nB;
+    ctx->gradient_stream = gradient_stream;
     bool ok = hip_ok(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (ok && gradient_stream == 2)
+        ok = hip_ok(ctx, hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking), "hipStreamCreate(gradient)") &&
+             hip_ok(ctx, hipEventCreateWithFlags(&ctx->sdone, hipEventDisableTiming), "hipEventCreate") &&
+             hip_ok(ctx, hipEventCreateWithFlags(&ctx->gdone[0], hipEventDisableTiming), "hipEventCreate") &&
+             hip_ok(ctx, hipEventCreateWithFlags(&ctx->gdone[1], hipEventDisableTiming), "hipEventCreate");
     ok = ok && dalloc(ctx, ctx->q[0], 5 * n, "q0") && dalloc(ctx, ctx->q[1], 5 * n, "q1");
     ok = ok && dalloc(ctx, ctx->res, 5 * (size_t)L.nC, "res") && dalloc(ctx, ctx->dt, L.nC, "dt") &&
          dalloc(ctx, ctx->diag, L.nC, "diag") && dalloc(ctx, ctx->dq, 5 * n, "dq") &&
@@ -821,6 +863,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     if (L.tri_slots) ok = ok && dalloc(ctx, ctx->snode, 12 * (size_t)L.nC, "slot nodes");
     ok = ok && dalloc(ctx, ctx->sdn, L.cfs.size(), "slot distances") && dalloc(ctx, ctx->sflip, L.nC, "slot sides");
     ok = ok && ensure_hist(ctx, 64);
+    // per-device set-up: the sweep order's event
+    ok = ok && sweep_order_setup(ctx->device);
     if (ok) {
         // resident workgroups of the sweep launch
         int per_cu = 0, cus = 0;
@@ -1015,6 +1059,65 @@ extern "C" int mfs_iteration_done(mfs_ctx* ctx) {
     return MFS_OK;
 }
 
+namespace {
+// The iterations of mfs_step; the history records go to ctx->hist from
+// slot ctx->hist_n on, which must hold nsteps more records.
+int step_iterations(mfs_ctx* ctx, int nsteps) {
+    // The gradient feeds nothing in the iteration (CalculateGradient's
+    // output is not read by the flux, time step or sweeps), so it may run
+    // beside the next iteration; only the state buffer it reads must
+    // outlive it.
+    const bool side = ctx->gradient_stream == 2 && ctx->gstream;
+    const bool fuse = !side && gradient_fuses_next(ctx);
+    for (int s = 0; s < nsteps; ++s) {
+        const double cfl = cfl_of_iter(ctx->P, ctx->iter_done);
+        const double cfl_min = 0.5 * cfl;
+        ctx->cfl_min = cfl_min;
+        // ZeroResiduals + InviscidFlux + ComputeTimeStep with the LU-SGS
+        // diagonal, in one pass over the faces (res written from zero);
+        // after the first iteration the previous gradient launch did it
+        // when fused.
+        if ((s == 0 || !fuse) && !launch_flux_ts(ctx, cfl, cfl_min)) return MFS_ERR_HIP;
+        if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
+        if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
+        // The backward sweep writes the state buffer that the gradient of
+        // two iterations back reads.
+        const int par = ctx->iter_done & 1;
+        if (side && ctx->gpend[par]) {
+            if (!hip_ok(ctx, hipStreamWaitEvent(ctx->stream, ctx->gdone[par], 0), "wait gradient")) return MFS_ERR_HIP;
+            ctx->gpend[par] = false;
+        }
+        // ForwardLUSGS: sweeps + update
+        if (!launch_sweeps(ctx)) return MFS_ERR_HIP;
+        // SetGhostVariables, when enabled (main.cpp:283 has the call commented out)
+        if (ctx->ghost_refresh && !launch_ghosts(ctx)) return MFS_ERR_HIP;
+        // CalculateGradient (fused: with the next iteration's first pass)
+        if (side) {
+            if (!hip_ok(ctx, hipEventRecord(ctx->sdone, ctx->stream), "record update") ||
+                !hip_ok(ctx, hipStreamWaitEvent(ctx->gstream, ctx->sdone, 0), "gradient waits update") ||
+                !launch_gradient(ctx, false, 0.0, ctx->gstream) ||
+                !hip_ok(ctx, hipEventRecord(ctx->gdone[par], ctx->gstream), "record gradient"))
+                return MFS_ERR_HIP;
+            ctx->gpend[par] = true;
+        } else {
+            const bool next = fuse && s + 1 < nsteps;
+            if (!launch_gradient(ctx, next, next ? cfl_of_iter(ctx->P, ctx->iter_done + 1) : 0.0)) return MFS_ERR_HIP;
+        }
+        ctx->hist_n++;
+        ctx->iter_done++;
+    }
+    if (side && nsteps > 0) {
+        // join: everything later on the solver stream (entry points,
+        // downloads, mfs_sync) comes after the last gradient
+        const int last = (ctx->iter_done - 1) & 1;
+        if (!hip_ok(ctx, hipStreamWaitEvent(ctx->stream, ctx->gdone[last], 0), "join gradient")) return MFS_ERR_HIP;
+        ctx->gpend[0] = ctx->gpend[1] = false;
+    }
+    return MFS_OK;
+}
+
+}  // namespace
+
 extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
     if (!usable(ctx)) return MFS_ERR_STATE;
     if (nsteps < 0) {
@@ -1022,28 +1125,7 @@ extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
         return MFS_ERR_ARG;
     }
     if (!ensure_hist(ctx, ctx->hist_n + nsteps)) return MFS_ERR_HIP;
-    const bool fuse = gradient_fuses_next(ctx);
-    for (int s = 0; s < nsteps; ++s) {
-        const double cfl = cfl_of_iter(ctx->P, ctx->iter_done);
-        const double cfl_min = 0.5 * cfl;
-        ctx->cfl_min = cfl_min;
-        // ZeroResiduals + InviscidFlux + ComputeTimeStep with the LU-SGS
-        // diagonal, in one pass over the faces (res written from zero);
-        // after the first iteration the previous gradient launch did it.
-        if ((s == 0 || !fuse) && !launch_flux_ts(ctx, cfl, cfl_min)) return MFS_ERR_HIP;
-        if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
-        if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
-        // ForwardLUSGS: sweeps + update
-        if (!launch_sweeps(ctx)) return MFS_ERR_HIP;
-        // SetGhostVariables, when enabled (main.cpp:283 has the call commented out)
-        if (ctx->ghost_refresh && !launch_ghosts(ctx)) return MFS_ERR_HIP;
-        // CalculateGradient (with the next iteration's first pass)
-        const bool next = fuse && s + 1 < nsteps;
-        if (!launch_gradient(ctx, next, next ? cfl_of_iter(ctx->P, ctx->iter_done + 1) : 0.0)) return MFS_ERR_HIP;
-        ctx->hist_n++;
-        ctx->iter_done++;
-    }
-    return MFS_OK;
+    return step_iterations(ctx, nsteps);
 }
 
 extern "C" int mfs_sync(mfs_ctx* ctx) {
@@ -1619,6 +1701,10 @@ extern "C" int mfs_batch_create(int precision, mfs_batch** out) {
     mfs_batch* b = new mfs_batch();
     b->precision = precision;
     (void)hipGetDevice(&b->device);
+    if (!sweep_order_setup(b->device)) {
+        delete b;
+        return MFS_ERR_HIP;
+    }
     const size_t tbytes = sizeof(int) * 2 * kTicketGroups * kTicketStride;
     if (!batch_ok(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         !batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate") ||

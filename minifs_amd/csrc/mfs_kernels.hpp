@@ -101,6 +101,10 @@ __global__ void k_fill_f(float* p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+__global__ void k_fill_i(int* p, size_t n, int v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
 // SetGhostVariables (src/Kernels/load_grid.cpp:953-1212), inviscid branch:
 // the ghost state behind each boundary face from the state of the cell in

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <numeric>
 #include <sstream>
+#include <thread>
 
 #include "mfs_mesh.hpp"
 
@@ -13,7 +14,30 @@ namespace mfs {
 
 namespace {
 constexpr double kTINY_ = 1.e-40;
+
+// Runs fn(begin, end, t) over T contiguous ranges of [0, n) on T threads
+// (the calling thread takes the last range).  Every result below is written
+// by exactly one range and does not depend on T.
+int layout_threads(size_t n) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t want = n / 65536 + 1;
+    return (int)std::max<size_t>(1, std::min<size_t>({want, hw ? hw : 1, 16}));
 }
+template <typename F>
+void parallel_ranges(size_t n, int T, F&& fn) {
+    std::vector<std::thread> th;
+    th.reserve(T > 0 ? T - 1 : 0);
+    for (int t = 0; t + 1 < T; ++t) th.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T, t); });
+    fn(n * (T - 1) / T, n, T - 1);
+    for (auto& x : th) x.join();
+}
+template <typename F>
+void parallel_for(size_t n, F&& fn) {
+    parallel_ranges(n, layout_threads(n), [&](size_t b, size_t e, int) {
+        for (size_t i = b; i < e; ++i) fn(i);
+    });
+}
+}  // namespace
 
 std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) {
     if (kTileCells < 64) return "tile size must be at least 64 cells";
@@ -138,9 +162,11 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
     L.cfo.assign(nC + 1, 0);
     L.cf.resize(mcf.size());
     L.depmask.assign(nC, 0);
-    {
-        int pos = 0;
-        for (int d = 0; d < nC; ++d) {
+    for (int d = 0; d < nC; ++d) L.cfo[d + 1] = L.cfo[d] + (mcfo[L.cell_of_dev[d] + 1] - mcfo[L.cell_of_dev[d]]);
+    parallel_for(nC, [&](size_t dd) {
+        {
+            const int d = (int)dd;
+            int pos = L.cfo[d];
             const int c = L.cell_of_dev[d];
             int s = 0;
             unsigned mask = 0;
@@ -162,9 +188,8 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
                 }
             }
             L.depmask[d] = (int)mask;
-            L.cfo[d + 1] = pos;
         }
-    }
+    });
 
     // ---- device node order: first touch over the device faces
     std::vector<int> dnode(nN, -1);
@@ -198,9 +223,10 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
     L.dn2.assign(nF, 0.0);
     L.fno.resize(nF + 1);
     L.fno[0] = 0;
-    L.fn.clear();
-    L.fn.reserve(mfo[nF]);
-    for (int d = 0; d < nF; ++d) {
+    for (int d = 0; d < nF; ++d) L.fno[d + 1] = L.fno[d] + v.nNPF[L.face_of_dev[d]];
+    L.fn.resize(mfo[nF]);
+    parallel_for(nF, [&](size_t dd) {
+        const int d = (int)dd;
         const int i = L.face_of_dev[d];
         const int a = v.f2c[2 * i], b = v.f2c[2 * i + 1];
         L.c1[d] = dmap(a);
@@ -216,11 +242,10 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
         if (i >= nB)
             L.dn2[d] = fabs((v.xfc[i] - v.xcc[b]) * v.xfn[i] + (v.yfc[i] - v.ycc[b]) * v.yfn[i] +
                             (v.zfc[i] - v.zcc[b]) * v.zfn[i]);
-        for (int j = mfo[i]; j < mfo[i + 1]; ++j) L.fn.push_back(dnode[v.f2n[j]]);
-        L.fno[d + 1] = (int)L.fn.size();
-    }
+        for (int j = mfo[i], o = L.fno[d]; j < mfo[i + 1]; ++j, ++o) L.fn[o] = dnode[v.f2n[j]];
+    });
     L.vol.resize(nC);
-    for (int d = 0; d < nC; ++d) L.vol[d] = v.vol[L.cell_of_dev[d]];
+    parallel_for(nC, [&](size_t d) { L.vol[d] = v.vol[L.cell_of_dev[d]]; });
 
     // ---- node weights (ComputeWeight3D_Node, grid_polyhedra_auxiliary_functions.cpp:831-1049)
     // and the accumulation order of CompNodeVar3D_dist (cal_gradient.cpp:1922-2372),
@@ -286,17 +311,33 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
             for (int j = mfo[i]; j < mfo[i + 1]; j++)
                 if (face_entry_ok(v.f2n[j], loop)) ecount[v.f2n[j]]++;
         }
-    std::vector<int> cstamp(nN, -1);
-    for (int c = 0; c < nC; ++c)
+    // The cell loop on threads, each over a contiguous range of cells: a
+    // node's entries come in ascending cell id, so thread t's entries of
+    // node p follow those of threads < t (a counting sort per node).
+    auto cell_nodes = [&](int c, std::vector<int>& nodes) {  // the cell's distinct nodes, first-visit order
+        nodes.clear();
         for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
             const int f = mcf[k];
             for (int j = mfo[f]; j < mfo[f + 1]; ++j) {
                 const int p1 = v.f2n[j];
-                if (cstamp[p1] == c) continue;
-                cstamp[p1] = c;
-                if (Nmark[p1] == 0) ecount[p1]++;
+                if (std::find(nodes.begin(), nodes.end(), p1) == nodes.end()) nodes.push_back(p1);
             }
         }
+    };
+    const int TC = layout_threads(nC);
+    std::vector<std::vector<int>> tcount(TC);  // per thread: interior-node entries per node, then fill offsets
+    parallel_ranges(nC, TC, [&](size_t b, size_t e, int t) {
+        std::vector<int>& cnt = tcount[t];
+        cnt.assign(nN, 0);
+        std::vector<int> nodes;
+        for (int c = (int)b; c < (int)e; ++c) {
+            cell_nodes(c, nodes);
+            for (int p1 : nodes)
+                if (Nmark[p1] == 0) cnt[p1]++;
+        }
+    });
+    for (int t = 0; t < TC; ++t)
+        for (int p1 = 0; p1 < nN; ++p1) ecount[p1] += tcount[t][p1];
     L.nlo.assign(nN + 1, 0);
     for (int d = 0; d < nN; ++d) L.nlo[d + 1] = L.nlo[d] + ecount[L.node_of_dev[d]];
     L.nla.resize(L.nlo[nN]);
@@ -316,28 +357,46 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
                 L.nlw[e] = wbf[mfo[i] + j];
             }
         }
-    std::fill(cstamp.begin(), cstamp.end(), -1);
-    for (int c = 0; c < nC; ++c)
-        for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
-            const int f = mcf[k];
-            for (int j = mfo[f]; j < mfo[f + 1]; ++j) {
-                const int p1 = v.f2n[j];
-                if (cstamp[p1] == c) continue;
-                cstamp[p1] = c;
+    // each thread's first entry of every interior node
+    parallel_for(nN, [&](size_t pp) {
+        int o = efill[pp];
+        for (int t = 0; t < TC; ++t) {
+            const int k = tcount[t][pp];
+            tcount[t][pp] = o;
+            o += k;
+        }
+    });
+    parallel_ranges(nC, TC, [&](size_t b, size_t e, int t) {
+        std::vector<int>& off = tcount[t];
+        std::vector<int> nodes;
+        for (int c = (int)b; c < (int)e; ++c) {
+            cell_nodes(c, nodes);
+            for (int p1 : nodes) {
                 if (Nmark[p1] != 0) continue;
                 const double dx = v.x[p1] - v.xcc[c], dy = v.y[p1] - v.ycc[c], dz = v.z[p1] - v.zcc[c];
                 double wr = dx * dx + dy * dy + dz * dz;
                 wr = sqrt(wr);
                 wr = 1. / wr;
-                wnode[p1] += wr;
-                const int e = efill[p1]++;
-                L.nla[e] = dcell[c];
-                L.nlb[e] = -1;
-                L.nlw[e] = wr;
+                const int en = off[p1]++;
+                L.nla[en] = dcell[c];
+                L.nlb[en] = -1;
+                L.nlw[en] = wr;
             }
         }
+    });
+    tcount.clear();
+    // WeightNode of an interior node: its cell weights summed in ascending
+    // cell id, the order of the reference's loop (:1007-1044)
     L.nden.resize(nN);
-    for (int d = 0; d < nN; ++d) L.nden[d] = wnode[L.node_of_dev[d]] + kTINY_;
+    parallel_for(nN, [&](size_t dd) {
+        const int pp = L.node_of_dev[dd];
+        if (Nmark[pp] == 0) {
+            double w = 0.0;
+            for (int en = L.nlo[dd]; en < L.nlo[dd + 1]; ++en) w += L.nlw[en];
+            wnode[pp] = w;
+        }
+        L.nden[dd] = wnode[pp] + kTINY_;
+    });
 
     // FindNodeSYMM (grid_polyhedra_auxiliary_functions.cpp:2060-2140)
     {
@@ -402,43 +461,53 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
         }
         const int nchunk = (int)(L.chunk.size() / 2);
         L.tile_chunk.push_back(nchunk);
-        // flat schedule: every chunk of level l before any of level l+1
+        // flat schedule: every chunk of level l before any of level l+1,
+        // chunks of one level in chunk order (a stable counting sort)
         L.run.resize(nchunk);
-        std::iota(L.run.begin(), L.run.end(), 0);
-        std::stable_sort(L.run.begin(), L.run.end(), [&](int a, int b) {
-            return L.flevel[L.cell_of_dev[L.chunk[2 * a]]] < L.flevel[L.cell_of_dev[L.chunk[2 * b]]];
-        });
+        {
+            std::vector<int> at(L.nlev + 1, 0);
+            for (int t = 0; t < nchunk; ++t) at[L.flevel[L.cell_of_dev[L.chunk[2 * t]]] + 1]++;
+            for (int l = 0; l < L.nlev; ++l) at[l + 1] += at[l];
+            for (int t = 0; t < nchunk; ++t) L.run[at[L.flevel[L.cell_of_dev[L.chunk[2 * t]]]]++] = t;
+        }
         L.rep_f.assign(nchunk, -1);
         L.rep_b.assign(nchunk, -1);
-        // rank of a dependency: level first, then "in another tile"
-        std::vector<long long> bestF(nchunk, -1), bestB(nchunk, -1);
-        for (int d = 0; d < nC; ++d) {
-            const int c = L.cell_of_dev[d];
-            const int t = chunk_of[d];
-            for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
-                const int f = mcf[k];
-                if (f < nB) continue;
-                const int nb = other(c, f);
-                const int dn = dcell[nb];
-                const int tn = chunk_of[dn];
-                const int foreign = (nb / kTileCells != c / kTileCells) ? 1 : 0;
-                if (nb < c) {
-                    if (tn >= t) return std::string("forward sweep chunks are not topological");
-                    const long long r = 2LL * L.flevel[nb] + foreign;
-                    if (r > bestF[t]) {
-                        bestF[t] = r;
-                        L.rep_f[t] = dn;
-                    }
-                } else {
-                    if (tn <= t) return std::string("backward sweep chunks are not topological");
-                    const long long r = 2LL * (L.nlev - L.flevel[nb]) + foreign;
-                    if (r > bestB[t]) {
-                        bestB[t] = r;
-                        L.rep_b[t] = dn;
+        // rank of a dependency: level first, then "in another tile"; the
+        // first of equal rank in the chunk's cell and slot order
+        std::vector<char> bad(nchunk, 0);
+        parallel_for(nchunk, [&](size_t tt) {
+            const int t = (int)tt;
+            long long bestF = -1, bestB = -1;
+            for (int d = L.chunk[2 * t]; d < L.chunk[2 * t] + L.chunk[2 * t + 1]; ++d) {
+                const int c = L.cell_of_dev[d];
+                for (int k = mcfo[c]; k < mcfo[c + 1]; ++k) {
+                    const int f = mcf[k];
+                    if (f < nB) continue;
+                    const int nb = other(c, f);
+                    const int dn = dcell[nb];
+                    const int tn = chunk_of[dn];
+                    const int foreign = (nb / kTileCells != c / kTileCells) ? 1 : 0;
+                    if (nb < c) {
+                        if (tn >= t) bad[t] = 1;
+                        const long long r = 2LL * L.flevel[nb] + foreign;
+                        if (r > bestF) {
+                            bestF = r;
+                            L.rep_f[t] = dn;
+                        }
+                    } else {
+                        if (tn <= t) bad[t] = 2;
+                        const long long r = 2LL * (L.nlev - L.flevel[nb]) + foreign;
+                        if (r > bestB) {
+                            bestB = r;
+                            L.rep_b[t] = dn;
+                        }
                     }
                 }
             }
-        }
+        });
+        for (int t = 0; t < nchunk; ++t)
+            if (bad[t])
+                return std::string(bad[t] == 1 ? "forward" : "backward") + " sweep chunks are not topological";
         // face slots per device cell (device face ids, ascending mesh face id, -1 = none)
         L.cfs.assign((size_t)nC * L.lpc, -1);
         for (int d = 0; d < nC; ++d)
@@ -480,10 +549,11 @@ std::string build_layout(const mfs_mesh_view& v, HostLayout& L, int kTileCells) 
     // ---- state, device order (ghosts unchanged)
     L.q.resize(5 * (size_t)n);
     const double* src[5] = {v.rho, v.u, v.v, v.w, v.p};
-    for (int j = 0; j < 5; ++j) {
-        for (int d = 0; d < nC; ++d) L.q[(size_t)j * n + d] = src[j][L.cell_of_dev[d]];
+    parallel_for(nC, [&](size_t d) {
+        for (int j = 0; j < 5; ++j) L.q[(size_t)j * n + d] = src[j][L.cell_of_dev[d]];
+    });
+    for (int j = 0; j < 5; ++j)
         for (int k = nC; k < n; ++k) L.q[(size_t)j * n + k] = src[j][k];
-    }
     return "";
 }
 

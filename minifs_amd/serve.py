@@ -17,6 +17,10 @@ server overlaps three things:
 Each solve's arithmetic is the single-solve path's, so a result is the same
 alone and in the mix.  `run_sequential` is the baseline: the same requests
 one at a time, each set up and solved alone on the plain FP64 path.
+
+Both build every context with the layout options the autotuner measured for
+the nearest mesh size (`tune.tuned_options`, tile size and sweep workgroup
+size); the options change the device order and the schedule, never a result.
 """
 from __future__ import annotations
 
@@ -29,6 +33,12 @@ import time
 import numpy as np
 
 from . import MFS_FP64, Batch, Grid, Solver, make_params
+from .tune import tuned_options
+
+
+def tuned_solver(g: Grid, p, precision: int) -> Solver:
+    """A context with the tuned layout options for this mesh size."""
+    return Solver(g, p, precision=precision, **tuned_options(g.info["nTCell"], precision))
 
 
 @dataclasses.dataclass
@@ -173,7 +183,7 @@ class Server:
                         break
                     ready.pop(0)
                     used += need
-                    creating[ctx_pool.submit(Solver, g, p, self.precision)] = (r, g, need)
+                    creating[ctx_pool.submit(tuned_solver, g, p, self.precision)] = (r, g, need)
                 # built contexts join the batch (cheap: no host layout work here)
                 for fut in [f for f in creating if f.done()]:
                     r, g, need = creating.pop(fut)
@@ -231,7 +241,7 @@ def run_sequential(requests: list[Request], drop: float = 1e-8, max_iters: int =
             time.sleep(wait)
         g, p = _setup(r, cfl, tree, device_metrics)
         r.t_ready = r.t_admit = now()
-        s = Solver(g, p, precision=precision)
+        s = tuned_solver(g, p, precision)
         r.n_cells = g.info["nTCell"]
         while True:
             s.step(check_every)

@@ -30,3 +30,13 @@ def ref_driver():
     if not os.path.exists(p):
         pytest.skip("oracle/_ref/ref_driver not built")
     return p
+
+
+@pytest.fixture(scope="session")
+def shim_driver():
+    """Path of the reference's set-up and driver loop with the solver entry
+    points of minifs_amd/shim on libmfs_hip.so (oracle/_ref/shim_driver), or skip."""
+    p = os.path.join(ROOT, "oracle", "_ref", "shim_driver")
+    if not os.path.exists(p):
+        pytest.skip("oracle/_ref/shim_driver not built")
+    return p

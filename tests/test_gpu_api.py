@@ -168,6 +168,39 @@ def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles)
     b.close()
 
 
+@pytest.mark.parametrize("cells,precision,ghosts", [("tet", mfs.MFS_FP64, False), ("tet", mfs.MFS_MIXED, False),
+                                                     ("tet", mfs.MFS_FP64, True), ("hex", mfs.MFS_FP64, False),
+                                                     ("prism", mfs.MFS_MIXED, True)])
+def test_gradient_on_second_stream_same_results(cells, precision, ghosts):
+    """mfs_step with the gradient on a second stream (beside the next
+    iteration's residual pass and sweeps) gives the same bits as with the
+    gradient on the solver stream: every field and every history record,
+    across several mfs_step calls and with kernel entry points in between."""
+    print(f"[api] gradient stream 2 vs 1: {cells} precision={precision} ghosts={ghosts}", flush=True)
+    p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
+    g = mfs.Grid.box_bump(10, cells=cells).decouple().preprocess(p)
+    a = mfs.Solver(g, p, precision=precision, gradient_stream=1)
+    b = mfs.Solver(g, p, precision=precision, gradient_stream=2)
+    for s in (a, b):
+        s.ghost_refresh(ghosts)
+        s.step(7)
+        s.step(1)
+        s.zero_residuals()
+        s.inviscid_flux()
+        s.compute_timestep()
+        s.forward_lusgs()
+        s.calculate_gradient()
+        s.iteration_done()
+        s.step(5)
+        s.sync()
+    for nm in FIELDS:
+        np.testing.assert_array_equal(a.field(nm), b.field(nm), err_msg=nm)
+    np.testing.assert_array_equal(a.history(), b.history())
+    a.close()
+    b.close()
+    g.close()
+
+
 @pytest.mark.parametrize("entry_points,setup", [(False, ""), (True, ""), (False, "tree+device")])
 def test_native_driver_matches_python_api(entry_points, setup):
     """mfs_solve (the native driver of the reference's main loop) gives the
