@@ -168,9 +168,10 @@ typedef struct mfs_ctx_options {
     int sweep_waves;    /* waves per sweep workgroup: 4, 8 or 16 (default 4) */
     int sweep_by_tiles; /* 0: flat level-order schedule (default); 1: one workgroup per tile */
     int gradient_stream; /* mfs_step's CalculateGradient: 1 on the context stream, fused with the next
-                          * residual pass (default); 2 on a second stream of the context, beside the
-                          * next iteration's residual pass and sweeps (nothing in an iteration reads
-                          * the gradient) */
+                          * residual pass; 2, 3, 4 on a second stream of the context, started after the
+                          * update, after the next iteration's residual pass, or after its forward sweep
+                          * (nothing in an iteration reads the gradient).  Default 3 for MFS_FP64, 4 for
+                          * MFS_MIXED (the fastest on 6.0M cells). */
 } mfs_ctx_options;
 int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p, int precision, const mfs_ctx_options* opt,
                       mfs_ctx** out);
@@ -263,6 +264,14 @@ int mfs_ctx_reset_timing(mfs_ctx* ctx);
  * (tetrahedral meshes, or meshes with up to 6 faces per cell).          */
 typedef struct mfs_batch mfs_batch;
 int mfs_batch_create(int precision, mfs_batch** out);
+/* Options of a batch (no counterpart in the reference); zero fields take
+ * the defaults.  Results do not depend on them. */
+typedef struct mfs_batch_options {
+    int gradient_stream; /* the batch step's CalculateGradient: 1 on the batch stream, fused with the next
+                          * residual pass (default); 2 or 4 on a second stream, started after the update or
+                          * after the next iteration's forward sweep */
+} mfs_batch_options;
+int mfs_batch_create_ex(int precision, const mfs_batch_options* opt, mfs_batch** out);
 int mfs_batch_destroy(mfs_batch* b);
 /* Builds the solve's device layout and uploads it; *slot names it until
  * removed.  mfs_batch_ctx(b, slot) gives its context for mfs_get_history,

@@ -266,9 +266,10 @@ class _BatchMember(Solver):
 class Batch:
     """Several independent solves stepped together (mfs_batch_*)."""
 
-    def __init__(self, precision: int = MFS_FP64):
+    def __init__(self, precision: int = MFS_FP64, gradient_stream: int = 0):
         self._h = C.c_void_p()
-        check(lib().mfs_batch_create(precision, C.byref(self._h)), "mfs_batch_create")
+        opt = _lib.MfsBatchOptions(gradient_stream)
+        check(lib().mfs_batch_create_ex(precision, C.byref(opt), C.byref(self._h)), "mfs_batch_create_ex")
         self.members: dict[int, _BatchMember] = {}
 
     def add(self, grid: Grid, params: _lib.MfsParams) -> _BatchMember:

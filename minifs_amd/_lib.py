@@ -67,6 +67,10 @@ class MfsCtxOptions(C.Structure):
                 ("gradient_stream", C.c_int)]
 
 
+class MfsBatchOptions(C.Structure):
+    _fields_ = [("gradient_stream", C.c_int)]
+
+
 # Every symbol declared in include/mfs.h, with its ctypes signature.
 _SIGS = {
     "mfs_params_default": (None, [C.POINTER(MfsParams)]),
@@ -113,6 +117,7 @@ _SIGS = {
     "mfs_ctx_get_timing": (C.c_int, [C.c_void_p, C.c_char_p, _DP, _IP]),
     "mfs_ctx_reset_timing": (C.c_int, [C.c_void_p]),
     "mfs_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mfs_batch_create_ex": (C.c_int, [C.c_int, C.POINTER(MfsBatchOptions), C.POINTER(C.c_void_p)]),
     "mfs_batch_destroy": (C.c_int, [C.c_void_p]),
     "mfs_batch_add": (C.c_int, [C.c_void_p, C.POINTER(MfsMeshView), C.POINTER(MfsParams), C.POINTER(C.c_int)]),
     "mfs_batch_attach": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),

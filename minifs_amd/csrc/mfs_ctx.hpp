@@ -101,7 +101,7 @@ struct mfs_ctx {
     hipStream_t gstream = nullptr;
     hipEvent_t sdone = nullptr, gdone[2] = {nullptr, nullptr};
     bool gpend[2] = {false, false};
-    int gradient_stream = 1;  // 1: on the solver stream (fused with the next residual pass), 2: second stream
+    int gradient_stream = 1;  // 1: on the solver stream (fused with the next residual pass), 2-4: second stream
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;
     int nlev = 0;

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <mutex>
 #include <sstream>
 #include <vector>
@@ -64,9 +65,16 @@ constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wa
 constexpr int kTicketGroups = 64;
 constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
 // Where mfs_step runs CalculateGradient: 1 on the solver stream, fused with
-// the next iteration's residual pass; 2 on a second stream, beside the next
-// iteration's residual pass and sweeps.
-constexpr int kDefaultGradientStream = 1;
+// the next iteration's residual pass; 2, 3, 4 on a second stream, started
+// after the update, after the next residual pass, after the next forward
+// sweep.
+// Measured on 6.0M cells (ms per iteration, modes 1/2/3/4): FP64
+// 4.47/4.51/4.23/4.50, mixed precision 3.63/3.66/3.55/3.34.
+constexpr int kDefaultGradientStreamFP64 = 3;
+constexpr int kDefaultGradientStreamMixed = 4;
+// The same for the batch step (1: fused with the next residual pass; 2, 4:
+// second stream after the update / after the next forward sweep).
+constexpr int kDefaultBatchGradientStream = 1;
 
 #include "mfs_kernels.hpp"
 #include "mfs_sweep.hpp"
@@ -491,7 +499,7 @@ bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
 }
 
 template <int LPC, typename R>
-bool launch_sweeps_t(mfs_ctx* ctx) {
+bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after_forward) {
     dev::SweepArgs<R> A;
     A.nC = ctx->nC;
     A.nchunks = ctx->nchunk;
@@ -534,11 +542,11 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
         A.tickets_next = tk_b;
         void* args[] = {&A, &ctx->K};
         if (!hip_ok(ctx,
-                    hipLaunchKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0,
-                                               ctx->stream),
+                    hipLaunchKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0, ctx->stream),
                     "k_sweep<fwd>"))
             return false;
     }
+    if (after_forward && !after_forward()) return false;
     {
         Timed t(ctx, "bwd_sweep");
         A.tickets = tk_b;
@@ -554,20 +562,24 @@ bool launch_sweeps_t(mfs_ctx* ctx) {
     ctx->cur = 1 - ctx->cur;
     return true;
 }
-bool launch_sweeps(mfs_ctx* ctx) {
+// after_forward, if set, is enqueued between the two sweeps.
+bool launch_sweeps(mfs_ctx* ctx, const std::function<bool()>& after_forward = nullptr) {
     if (ctx->precision == MFS_MIXED)
-        return ctx->lpc == 4 ? launch_sweeps_t<4, float>(ctx) : launch_sweeps_t<8, float>(ctx);
-    return ctx->lpc == 4 ? launch_sweeps_t<4, double>(ctx) : launch_sweeps_t<8, double>(ctx);
+        return ctx->lpc == 4 ? launch_sweeps_t<4, float>(ctx, after_forward)
+                             : launch_sweeps_t<8, float>(ctx, after_forward);
+    return ctx->lpc == 4 ? launch_sweeps_t<4, double>(ctx, after_forward)
+                         : launch_sweeps_t<8, double>(ctx, after_forward);
 }
 
 // CalculateGradient: node values, then the gradient.  With `next` set (the
 // cfl of the next iteration in next_cfl), a tetrahedral context launches
 // the gradient fused with the next iteration's residual / time-step pass.
 template <int MAXF, typename R>
-bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, hipStream_t st = nullptr) {
+bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, hipStream_t st = nullptr,
+                       const double* qs = nullptr) {
     if (!st) st = ctx->stream;
     const int n = ctx->nC + ctx->nB;
-    const double* q = ctx->q[ctx->cur];
+    const double* q = qs ? qs : ctx->q[ctx->cur];
     constexpr bool F64 = sizeof(R) == sizeof(double);
     R* qn;
     const R* ew;
@@ -646,12 +658,13 @@ bool launch_ghosts(mfs_ctx* ctx) {
     return check_launch(ctx, "k_ghost");
 }
 
-bool launch_gradient(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, hipStream_t st = nullptr) {
+bool launch_gradient(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, hipStream_t st = nullptr,
+                     const double* qs = nullptr) {
     if (ctx->precision == MFS_MIXED)
-        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx, next, next_cfl, st)
-                              : launch_gradient_t<6, float>(ctx, next, next_cfl, st);
-    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx, next, next_cfl, st)
-                          : launch_gradient_t<6, double>(ctx, next, next_cfl, st);
+        return ctx->maxf == 4 ? launch_gradient_t<4, float>(ctx, next, next_cfl, st, qs)
+                              : launch_gradient_t<6, float>(ctx, next, next_cfl, st, qs);
+    return ctx->maxf == 4 ? launch_gradient_t<4, double>(ctx, next, next_cfl, st, qs)
+                          : launch_gradient_t<6, double>(ctx, next, next_cfl, st, qs);
 }
 // whether launch_gradient(ctx, true, ...) also runs the next residual pass
 bool gradient_fuses_next(const mfs_ctx* ctx) { return ctx->maxf == 4 && ctx->snode; }
@@ -729,13 +742,15 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     const int tile_cells = (opt && opt->tile_cells) ? opt->tile_cells : kDefaultTileCells;
     const int sweep_waves = (opt && opt->sweep_waves) ? opt->sweep_waves : 4;
     const int sweep_by_tiles = (opt && opt->sweep_by_tiles) ? 1 : 0;
-    const int gradient_stream = (opt && opt->gradient_stream) ? opt->gradient_stream : dev::kDefaultGradientStream;
+    const int gradient_stream = (opt && opt->gradient_stream) ? opt->gradient_stream
+                                : precision == MFS_MIXED   ? dev::kDefaultGradientStreamMixed
+                                                           : dev::kDefaultGradientStreamFP64;
     if (tile_cells < 64 || (sweep_waves != 4 && sweep_waves != 8 && sweep_waves != 16)) {
         set_error("mfs_ctx_create: tile_cells must be >= 64 and sweep_waves 4, 8 or 16");
         return MFS_ERR_ARG;
     }
-    if (gradient_stream != 1 && gradient_stream != 2) {
-        set_error("mfs_ctx_create: gradient_stream must be 0 (default), 1 or 2");
+    if (gradient_stream < 1 || gradient_stream > 4) {
+        set_error("mfs_ctx_create: gradient_stream must be 0 (default) or 1 to 4");
         return MFS_ERR_ARG;
     }
     *out = nullptr;
@@ -794,7 +809,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     const size_t n = (size_t)L.nC + L.nB;
     ctx->gradient_stream = gradient_stream;
     bool ok = hip_ok(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
-    if (ok && gradient_stream == 2)
+    if (ok && gradient_stream >= 2)
         ok = hip_ok(ctx, hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking), "hipStreamCreate(gradient)") &&
              hip_ok(ctx, hipEventCreateWithFlags(&ctx->sdone, hipEventDisableTiming), "hipEventCreate") &&
              hip_ok(ctx, hipEventCreateWithFlags(&ctx->gdone[0], hipEventDisableTiming), "hipEventCreate") &&
@@ -1066,9 +1081,27 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
     // The gradient feeds nothing in the iteration (CalculateGradient's
     // output is not read by the flux, time step or sweeps), so it may run
     // beside the next iteration; only the state buffer it reads must
-    // outlive it.
-    const bool side = ctx->gradient_stream == 2 && ctx->gstream;
+    // outlive it.  gradient_stream 2/3/4: iteration k's node values and
+    // gradient start on the second stream after iteration k's update /
+    // iteration k+1's residual pass / iteration k+1's forward sweep.
+    const int mode = ctx->gstream ? ctx->gradient_stream : 1;
+    const bool side = mode >= 2;
     const bool fuse = !side && gradient_fuses_next(ctx);
+    const double* grad_q = nullptr;  // state of the gradient not yet launched
+    int grad_par = 0;
+    // records `ev` on the solver stream and starts the pending gradient
+    // behind it on the second stream
+    auto side_gradient = [&](hipEvent_t ev) -> bool {
+        if (!grad_q) return true;
+        if (!hip_ok(ctx, hipEventRecord(ev, ctx->stream), "record gradient start") ||
+            !hip_ok(ctx, hipStreamWaitEvent(ctx->gstream, ev, 0), "gradient waits") ||
+            !launch_gradient(ctx, false, 0.0, ctx->gstream, grad_q) ||
+            !hip_ok(ctx, hipEventRecord(ctx->gdone[grad_par], ctx->gstream), "record gradient"))
+            return false;
+        ctx->gpend[grad_par] = true;
+        grad_q = nullptr;
+        return true;
+    };
     for (int s = 0; s < nsteps; ++s) {
         const double cfl = cfl_of_iter(ctx->P, ctx->iter_done);
         const double cfl_min = 0.5 * cfl;
@@ -1078,6 +1111,7 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
         // after the first iteration the previous gradient launch did it
         // when fused.
         if ((s == 0 || !fuse) && !launch_flux_ts(ctx, cfl, cfl_min)) return MFS_ERR_HIP;
+        if (side && mode == 3 && !side_gradient(ctx->sdone)) return MFS_ERR_HIP;
         if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
         if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
         // The backward sweep writes the state buffer that the gradient of
@@ -1088,17 +1122,16 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
             ctx->gpend[par] = false;
         }
         // ForwardLUSGS: sweeps + update
-        if (!launch_sweeps(ctx)) return MFS_ERR_HIP;
+        std::function<bool()> after_fwd;
+        if (side && mode == 4 && grad_q) after_fwd = [&] { return side_gradient(ctx->sdone); };
+        if (!launch_sweeps(ctx, after_fwd)) return MFS_ERR_HIP;
         // SetGhostVariables, when enabled (main.cpp:283 has the call commented out)
         if (ctx->ghost_refresh && !launch_ghosts(ctx)) return MFS_ERR_HIP;
         // CalculateGradient (fused: with the next iteration's first pass)
         if (side) {
-            if (!hip_ok(ctx, hipEventRecord(ctx->sdone, ctx->stream), "record update") ||
-                !hip_ok(ctx, hipStreamWaitEvent(ctx->gstream, ctx->sdone, 0), "gradient waits update") ||
-                !launch_gradient(ctx, false, 0.0, ctx->gstream) ||
-                !hip_ok(ctx, hipEventRecord(ctx->gdone[par], ctx->gstream), "record gradient"))
-                return MFS_ERR_HIP;
-            ctx->gpend[par] = true;
+            grad_q = ctx->q[ctx->cur];
+            grad_par = par;
+            if ((mode == 2 || s + 1 == nsteps) && !side_gradient(ctx->sdone)) return MFS_ERR_HIP;
         } else {
             const bool next = fuse && s + 1 < nsteps;
             if (!launch_gradient(ctx, next, next ? cfl_of_iter(ctx->P, ctx->iter_done + 1) : 0.0)) return MFS_ERR_HIP;
@@ -1389,6 +1422,15 @@ struct mfs_batch {
     bool table_pending = false;
     int sweep_grid = 0;
     int* tickets = nullptr;       // ticket counters of the batch sweeps: forward, then backward
+    // gradient_stream 2 (after the update) / 4 (after the next forward
+    // sweep): the node values and gradient of a batch iteration on gstream;
+    // gdone[k & 1] after iteration k's, which reads the state buffers that
+    // the backward sweep of iteration k+2 overwrites
+    int gradient_stream = 1;
+    hipStream_t gstream = nullptr;
+    hipEvent_t gstart = nullptr, gdone[2] = {nullptr, nullptr};
+    bool gpend[2] = {false, false};
+    long long iters = 0;          // batch iterations stepped (parity of gdone)
 };
 
 namespace {
@@ -1554,6 +1596,22 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     (void)f64;
 }
 
+// Node values (with the symmetry projection) and the gradient of the
+// batch iteration whose table is T, on stream st.
+template <int MAXF, typename R>
+bool batch_gradient(mfs_batch* b, const dev::BatchSolve<R>* T, hipStream_t st) {
+    constexpr bool MIXED = sizeof(R) == sizeof(float);
+    hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, st, T, b->node_solve);
+    for (size_t a = 0; a < b->active.size(); ++a) {
+        mfs_ctx* c = b->slots[b->active[a]];
+        if (c->nsym > 0)
+            hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(c->nsym)), dim3(kBlock), 0, st, c->nsym, c->sym_node,
+                               c->sym_n, MIXED ? (R*)c->qn32 : (R*)c->qn);
+    }
+    hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, st, T, b->cell_solve);
+    return batch_ok(b, hipGetLastError(), "batch gradient");
+}
+
 template <int MAXF, int LPC, typename R>
 int batch_step_t(mfs_batch* b, int nsteps) {
     constexpr bool MIXED = sizeof(R) == sizeof(float);
@@ -1605,16 +1663,39 @@ int batch_step_t(mfs_batch* b, int nsteps) {
     auto* D = static_cast<dev::BatchSolve<R>*>(b->table_dev);
     const dim3 sweep_grid((unsigned)std::max(1, std::min(b->sweep_grid, (b->nitems + kBatchSweepWaves - 1) /
                                                                          kBatchSweepWaves)));
+    // The gradient feeds nothing in an iteration, so with a second stream it
+    // runs beside the next iteration (see step_iterations).
+    const int mode = b->gstream ? b->gradient_stream : 1;
+    const bool side = mode >= 2;
+    const dev::BatchSolve<R>* grad_T = nullptr;  // table of the gradient not yet launched
+    int grad_par = 0;
+    auto side_gradient = [&]() -> bool {
+        if (!grad_T) return true;
+        if (!batch_ok(b, hipEventRecord(b->gstart, b->stream), "record gradient start") ||
+            !batch_ok(b, hipStreamWaitEvent(b->gstream, b->gstart, 0), "gradient waits") ||
+            !batch_gradient<MAXF, R>(b, grad_T, b->gstream) ||
+            !batch_ok(b, hipEventRecord(b->gdone[grad_par], b->gstream), "record gradient"))
+            return false;
+        b->gpend[grad_par] = true;
+        grad_T = nullptr;
+        return true;
+    };
     for (int i = 0; i < nsteps; ++i) {
         const dev::BatchSolve<R>* Tf = D + (size_t)i * ns;
-        // after the first iteration the previous gradient launch did this pass
-        if (i == 0)
+        // after the first iteration the previous gradient launch did this
+        // pass when fused
+        if (i == 0 || side)
             hipLaunchKernelGGL((dev::k_b_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream,
                                Tf, b->cell_solve);
         hipLaunchKernelGGL((dev::k_b_reduce<R>), dim3(ns), dim3(dev::kReduceBlock), 0, b->stream, Tf);
         hipLaunchKernelGGL((dev::k_b_clamp<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
                            b->cell_solve);
         if (!batch_ok(b, hipGetLastError(), "batch flux/time step")) return MFS_ERR_HIP;
+        const int par = (int)(b->iters & 1);
+        if (side && b->gpend[par]) {
+            if (!batch_ok(b, hipStreamWaitEvent(b->stream, b->gdone[par], 0), "wait gradient")) return MFS_ERR_HIP;
+            b->gpend[par] = false;
+        }
         {
             SweepTurn turn(b->device, b->stream);
             if (turn.error()) return sweep_turn_failed(nullptr, turn.error()) ? MFS_OK : MFS_ERR_HIP;
@@ -1628,6 +1709,7 @@ int batch_step_t(mfs_batch* b, int nsteps) {
                                                      dim3(64 * kBatchSweepWaves), args, 0, b->stream),
                           "batch forward sweep"))
                 return MFS_ERR_HIP;
+            if (side && mode == 4 && !side_gradient()) return MFS_ERR_HIP;
             void* argsb[] = {(void*)&Tf, (void*)&b->items_b, (void*)&n_it, (void*)&tk_b, (void*)&tk_f, (void*)&ng};
             if (!batch_ok(b,
                           hipLaunchKernel((const void*)dev::k_b_sweep<false, LPC, R>, sweep_grid,
@@ -1647,20 +1729,33 @@ int batch_step_t(mfs_batch* b, int nsteps) {
                                    MIXED ? c->q32 : nullptr);
             }
         }
-        hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, b->stream, Tf, b->node_solve);
-        for (int a = 0; a < ns; ++a) {  // symmetry-plane projection, only where a solve has SYMM nodes
-            mfs_ctx* c = b->slots[b->active[a]];
-            if (c->nsym > 0)
-                hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(c->nsym)), dim3(kBlock), 0, b->stream, c->nsym,
-                                   c->sym_node, c->sym_n, MIXED ? (R*)c->qn32 : (R*)c->qn);
+        if (side) {
+            grad_T = Tf;
+            grad_par = par;
+            if ((mode == 2 || i + 1 == nsteps) && !side_gradient()) return MFS_ERR_HIP;
+        } else {
+            hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, b->stream, Tf,
+                               b->node_solve);
+            for (int a = 0; a < ns; ++a) {  // symmetry-plane projection, only where a solve has SYMM nodes
+                mfs_ctx* c = b->slots[b->active[a]];
+                if (c->nsym > 0)
+                    hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(c->nsym)), dim3(kBlock), 0, b->stream,
+                                       c->nsym, c->sym_node, c->sym_n, MIXED ? (R*)c->qn32 : (R*)c->qn);
+            }
+            if (i + 1 < nsteps)  // the gradient with the next iteration's residual / time-step pass
+                hipLaunchKernelGGL((dev::k_b_grad_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0,
+                                   b->stream, Tf, Tf + (size_t)ns, b->cell_solve);
+            else
+                hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
+                                   b->cell_solve);
+            if (!batch_ok(b, hipGetLastError(), "batch gradient")) return MFS_ERR_HIP;
         }
-        if (i + 1 < nsteps)  // the gradient with the next iteration's residual / time-step pass
-            hipLaunchKernelGGL((dev::k_b_grad_flux_ts<MAXF, MIXED, R>), dim3(b->ncell_blk), dim3(kBlock), 0,
-                               b->stream, Tf, Tf + (size_t)ns, b->cell_solve);
-        else
-            hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, b->stream, Tf,
-                               b->cell_solve);
-        if (!batch_ok(b, hipGetLastError(), "batch gradient")) return MFS_ERR_HIP;
+        ++b->iters;
+    }
+    if (side) {  // join: later work on the batch stream comes after the last gradient
+        const int last = (int)((b->iters - 1) & 1);
+        if (!batch_ok(b, hipStreamWaitEvent(b->stream, b->gdone[last], 0), "join gradient")) return MFS_ERR_HIP;
+        b->gpend[0] = b->gpend[1] = false;
     }
     for (int a = 0; a < ns; ++a) {
         mfs_ctx* c = b->slots[b->active[a]];
@@ -1687,9 +1782,17 @@ int batch_sweep_occupancy(int* per_cu) {
 
 }  // namespace
 
-extern "C" int mfs_batch_create(int precision, mfs_batch** out) {
+extern "C" int mfs_batch_create(int precision, mfs_batch** out) { return mfs_batch_create_ex(precision, nullptr, out); }
+
+extern "C" int mfs_batch_create_ex(int precision, const mfs_batch_options* opt, mfs_batch** out) {
     if (!out || (precision != MFS_FP64 && precision != MFS_MIXED)) {
         set_error("mfs_batch_create: null argument or unknown precision");
+        return MFS_ERR_ARG;
+    }
+    const int gradient_stream =
+        (opt && opt->gradient_stream) ? opt->gradient_stream : dev::kDefaultBatchGradientStream;
+    if (gradient_stream != 1 && gradient_stream != 2 && gradient_stream != 4) {
+        set_error("mfs_batch_create: gradient_stream must be 0 (default), 1, 2 or 4");
         return MFS_ERR_ARG;
     }
     *out = nullptr;
@@ -1700,18 +1803,27 @@ extern "C" int mfs_batch_create(int precision, mfs_batch** out) {
     }
     mfs_batch* b = new mfs_batch();
     b->precision = precision;
+    b->gradient_stream = gradient_stream;
     (void)hipGetDevice(&b->device);
     if (!sweep_order_setup(b->device)) {
         delete b;
         return MFS_ERR_HIP;
     }
     const size_t tbytes = sizeof(int) * 2 * kTicketGroups * kTicketStride;
-    if (!batch_ok(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") ||
-        !batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate") ||
-        !batch_ok(b, hipMalloc(&b->tickets, tbytes), "hipMalloc(batch tickets)") ||
-        !batch_ok(b, hipMemsetAsync(b->tickets, 0, tbytes, b->stream), "memset batch tickets")) {
+    bool ok = batch_ok(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") &&
+              batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate") &&
+              batch_ok(b, hipMalloc(&b->tickets, tbytes), "hipMalloc(batch tickets)") &&
+              batch_ok(b, hipMemsetAsync(b->tickets, 0, tbytes, b->stream), "memset batch tickets");
+    if (ok && gradient_stream >= 2)
+        ok = batch_ok(b, hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking), "hipStreamCreate(gradient)") &&
+             batch_ok(b, hipEventCreateWithFlags(&b->gstart, hipEventDisableTiming), "hipEventCreate") &&
+             batch_ok(b, hipEventCreateWithFlags(&b->gdone[0], hipEventDisableTiming), "hipEventCreate") &&
+             batch_ok(b, hipEventCreateWithFlags(&b->gdone[1], hipEventDisableTiming), "hipEventCreate");
+    if (!ok) {
         if (b->tickets) (void)hipFree(b->tickets);
-        if (b->table_copied) (void)hipEventDestroy(b->table_copied);
+        for (hipEvent_t e : {b->table_copied, b->gstart, b->gdone[0], b->gdone[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (b->gstream) (void)hipStreamDestroy(b->gstream);
         if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return MFS_ERR_HIP;
@@ -1864,7 +1976,9 @@ extern "C" int mfs_batch_destroy(mfs_batch* b) {
     (void)hipFree(b->tickets);
     (void)hipHostFree(b->table_host);
     (void)hipFree(b->table_dev);
-    (void)hipEventDestroy(b->table_copied);
+    for (hipEvent_t e : {b->table_copied, b->gstart, b->gdone[0], b->gdone[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (b->gstream) (void)hipStreamDestroy(b->gstream);
     forget_sweep_stream(b->device, b->stream);
     (void)hipStreamDestroy(b->stream);
     delete b;

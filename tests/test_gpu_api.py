@@ -168,19 +168,20 @@ def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles)
     b.close()
 
 
-@pytest.mark.parametrize("cells,precision,ghosts", [("tet", mfs.MFS_FP64, False), ("tet", mfs.MFS_MIXED, False),
-                                                     ("tet", mfs.MFS_FP64, True), ("hex", mfs.MFS_FP64, False),
-                                                     ("prism", mfs.MFS_MIXED, True)])
-def test_gradient_on_second_stream_same_results(cells, precision, ghosts):
-    """mfs_step with the gradient on a second stream (beside the next
-    iteration's residual pass and sweeps) gives the same bits as with the
-    gradient on the solver stream: every field and every history record,
-    across several mfs_step calls and with kernel entry points in between."""
-    print(f"[api] gradient stream 2 vs 1: {cells} precision={precision} ghosts={ghosts}", flush=True)
+@pytest.mark.parametrize("cells,precision,ghosts,mode", [("tet", mfs.MFS_FP64, False, 2), ("tet", mfs.MFS_MIXED, False, 3),
+                                                          ("tet", mfs.MFS_FP64, True, 4), ("hex", mfs.MFS_FP64, False, 4),
+                                                          ("prism", mfs.MFS_MIXED, True, 2), ("tet", mfs.MFS_MIXED, False, 4)])
+def test_gradient_on_second_stream_same_results(cells, precision, ghosts, mode):
+    """mfs_step with the gradient on a second stream (started after the
+    update, after the next residual pass or after the next forward sweep)
+    gives the same bits as with the gradient on the solver stream: every
+    field and every history record, across several mfs_step calls and with
+    kernel entry points in between."""
+    print(f"[api] gradient stream {mode} vs 1: {cells} precision={precision} ghosts={ghosts}", flush=True)
     p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
     g = mfs.Grid.box_bump(10, cells=cells).decouple().preprocess(p)
     a = mfs.Solver(g, p, precision=precision, gradient_stream=1)
-    b = mfs.Solver(g, p, precision=precision, gradient_stream=2)
+    b = mfs.Solver(g, p, precision=precision, gradient_stream=mode)
     for s in (a, b):
         s.ghost_refresh(ghosts)
         s.step(7)

@@ -47,6 +47,28 @@ def test_batch_members_equal_standalone(precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision,mode,cells", [(mfs.MFS_FP64, 2, "tet"), (mfs.MFS_MIXED, 4, "tet"),
+                                                  (mfs.MFS_MIXED, 2, "prism"), (mfs.MFS_FP64, 4, "hex")])
+def test_batch_gradient_on_second_stream_equals_standalone(precision, mode, cells):
+    """A batch whose gradient runs on a second stream (after the update, or
+    after the next iteration's forward sweep), with a solve joining and one
+    leaving between steps: every member equals the same solve alone."""
+    print(f"[batch] gradient stream {mode} precision={precision} {cells}", flush=True)
+    a, c, d = _case(7, 0.6, 1.0, 31, cells), _case(5, 0.7, 2.0, 32, cells), _case(6, 0.45, -1.0, 33, cells)
+    b = mfs.Batch(precision, gradient_stream=mode)
+    ma = b.add(*a)
+    mc = b.add(*c)
+    b.step(3)
+    md = b.add(*d)
+    b.step(2)
+    b.remove(mc)
+    b.step(4)
+    for m, case, iters in ((ma, a, 9), (md, d, 6)):
+        ref, href = _alone(*case, iters, precision)
+        _same(m, ref, href)
+    b.close()
+
+
 def test_batch_join_and_leave():
     a, c, d = _case(7, 0.6, 1.0, 11), _case(5, 0.7, 2.0, 12), _case(8, 0.45, -1.0, 13)
     b = mfs.Batch(mfs.MFS_FP64)
