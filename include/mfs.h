@@ -200,6 +200,20 @@ int mfs_iteration_done(mfs_ctx* ctx);
 /* n iterations of the driver loop (main.cpp:201-291), fused kernels;
  * appends n history records. */
 int mfs_step(mfs_ctx* ctx, int n);
+/* mfs_step for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops that forbid hipMalloc); it has no
+ * counterpart in the reference.  It calls no hipMalloc, hipFree or other
+ * allocating routine: the reduction partials and sweep ticket counters live
+ * in a static device array of the library (2.03 MiB per GPU, held until
+ * the process ends), and the n history records (7 doubles each, as
+ * mfs_get_history gives them) go to hist_dev, a device buffer of 7*n
+ * doubles that the caller owns.  Results are those of mfs_step(ctx, n);
+ * the records are not added to the context's own history and kernel
+ * timing is not recorded.  Limits: contexts of at most 8,388,608 cells
+ * (MFS_ERR_ARG above that), not a batch member.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (contexts stepped
+ * from several threads at once) race and must be serialised by the caller. */
+int mfs_step_noalloc(mfs_ctx* ctx, int n, double* hist_dev);
 int mfs_sync(mfs_ctx* ctx);
 
 /* Per-iteration history: record k = {norm_rho, norm_mx, norm_my, norm_mz,

@@ -105,6 +105,7 @@ _SIGS = {
     "mfs_calculate_gradient": (C.c_int, [C.c_void_p]),
     "mfs_iteration_done": (C.c_int, [C.c_void_p]),
     "mfs_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "mfs_step_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "mfs_sync": (C.c_int, [C.c_void_p]),
     "mfs_history_count": (C.c_int, [C.c_void_p]),
     "mfs_get_history": (C.c_int, [C.c_void_p, _DP, C.c_int]),

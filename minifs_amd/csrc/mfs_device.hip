@@ -30,6 +30,15 @@ namespace mfs {
 void set_error(const std::string& msg);  // mfs_capi.cpp
 }
 
+// The static scratch of mfs_step_noalloc: per-block partials of the
+// residual norms and time-step extrema for contexts of up to
+// kNoAllocMaxCells cells, then the two sweeps' ticket counters (2.03 MiB).
+constexpr int kNoAllocMaxCells = 1 << 23;
+constexpr size_t kNoAllocPartDoubles = (size_t)(((kNoAllocMaxCells + 255) / 256 + 7) / 8 * 8) * 8;
+constexpr size_t kNoAllocTicketInts = 2 * 64 * 64;
+constexpr size_t kNoAllocBytes = kNoAllocPartDoubles * sizeof(double) + kNoAllocTicketInts * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace mfs {
 namespace dev {
 
@@ -388,6 +397,49 @@ void forget_sweep_stream(int device, hipStream_t stream) {
     SweepOrder& o = g_sweep_order[device];
     std::lock_guard<std::mutex> lk(o.mu);
     if (o.last_stream == stream) o.last_stream = reinterpret_cast<hipStream_t>(~uintptr_t(0));
+}
+
+// ---------------------------------------------------------------------------
+// mfs_step_noalloc's scratch: the per-block partials of the residual norms
+// and time-step extrema (8 doubles per cell block) and the two sets of
+// sweep ticket counters, in the static device array `scratch` declared at
+// file scope below, sized for contexts of up to kNoAllocMaxCells cells
+// (2.03 MiB; every GPU the library runs on holds its own copy until the
+// process ends).  In host code `scratch` names a host-side stand-in: the
+// device address of each GPU's copy is taken once per device, with that
+// device current, by hipGetSymbolAddress at context creation and kept in
+// g_noalloc_scratch; only that address is passed to kernels.
+static_assert(kNoAllocPartDoubles ==
+                  (size_t)(((kNoAllocMaxCells + dev::kBlock - 1) / dev::kBlock + dev::kXcd - 1) / dev::kXcd * dev::kXcd) * 8,
+              "scratch partials: one 8-double record per cell block of the largest context");
+static_assert(kNoAllocTicketInts == 2 * (size_t)kTicketGroups * kTicketStride, "scratch tickets: both sweeps");
+
+std::mutex g_noalloc_mu;
+void* g_noalloc_scratch[kMaxDevices] = {};
+
+// With `device` current: takes the address of this device's copy of
+// `scratch`, once per device.
+bool noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device id out of range for the scratch table");
+        return false;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_scratch[device]) return true;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return false;
+    }
+    g_noalloc_scratch[device] = ptr;
+    return true;
+}
+
+void* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_scratch[device];
 }
 
 double cfl_of_iter(const mfs_params& P, int iter_done) {
@@ -878,8 +930,9 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     if (L.tri_slots) ok = ok && dalloc(ctx, ctx->snode, 12 * (size_t)L.nC, "slot nodes");
     ok = ok && dalloc(ctx, ctx->sdn, L.cfs.size(), "slot distances") && dalloc(ctx, ctx->sflip, L.nC, "slot sides");
     ok = ok && ensure_hist(ctx, 64);
-    // per-device set-up: the sweep order's event
-    ok = ok && sweep_order_setup(ctx->device);
+    // per-device set-up: the sweep order's event and the address of the
+    // static scratch of mfs_step_noalloc on this device
+    ok = ok && sweep_order_setup(ctx->device) && noalloc_scratch_setup(ctx->device);
     if (ok) {
         // resident workgroups of the sweep launch
         int per_cu = 0, cus = 0;
@@ -1149,6 +1202,35 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
     return MFS_OK;
 }
 
+// For the duration of one mfs_step_noalloc call: the context's partials,
+// ticket counters and history point into the static scratch and the
+// caller's record buffer, and kernel timing (which creates events) is off.
+struct NoAllocView {
+    mfs_ctx* ctx;
+    double* part;
+    int* tickets;
+    double* hist;
+    int hist_n, hist_cap;
+    bool timing;
+    NoAllocView(mfs_ctx* c, double* sp, int* st, double* h, int n)
+        : ctx(c), part(c->part), tickets(c->tickets), hist(c->hist), hist_n(c->hist_n), hist_cap(c->hist_cap),
+          timing(c->timing) {
+        c->part = sp;
+        c->tickets = st;
+        c->hist = h;
+        c->hist_n = 0;
+        c->hist_cap = n;
+        c->timing = false;
+    }
+    ~NoAllocView() {
+        ctx->part = part;
+        ctx->tickets = tickets;
+        ctx->hist = hist;
+        ctx->hist_n = hist_n;
+        ctx->hist_cap = hist_cap;
+        ctx->timing = timing;
+    }
+};
 }  // namespace
 
 extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
@@ -1158,6 +1240,39 @@ extern "C" int mfs_step(mfs_ctx* ctx, int nsteps) {
         return MFS_ERR_ARG;
     }
     if (!ensure_hist(ctx, ctx->hist_n + nsteps)) return MFS_ERR_HIP;
+    return step_iterations(ctx, nsteps);
+}
+
+extern "C" int mfs_step_noalloc(mfs_ctx* ctx, int nsteps, double* hist_dev) {
+    if (!usable(ctx)) return MFS_ERR_STATE;
+    if (nsteps < 0 || (nsteps > 0 && !hist_dev)) {
+        set_error("mfs_step_noalloc: negative step count or null history buffer");
+        return MFS_ERR_ARG;
+    }
+    if (!ctx->owns_stream) {
+        set_error("mfs_step_noalloc: the context belongs to a batch; step it with mfs_batch_step");
+        return MFS_ERR_STATE;
+    }
+    if (ctx->nC > kNoAllocMaxCells || (size_t)ctx->nblk_cell * 8 > kNoAllocPartDoubles) {
+        set_error("mfs_step_noalloc: " + std::to_string(ctx->nC) + " cells exceed the static scratch's limit of " +
+                  std::to_string(kNoAllocMaxCells) + " cells (use mfs_step)");
+        return MFS_ERR_ARG;
+    }
+    char* sp = static_cast<char*>(noalloc_scratch(ctx->device));
+    if (!sp) {
+        set_error("mfs_step_noalloc: the static scratch of this device is not set up");
+        return MFS_ERR_STATE;
+    }
+    double* part = reinterpret_cast<double*>(sp);
+    int* tickets = reinterpret_cast<int*>(sp + kNoAllocPartDoubles * sizeof(double));
+    // The counters start from zero (an earlier call may have ended with the
+    // backward ones in use); a fill kernel rather than a memset node, so the
+    // call can be captured in a graph.  The partials need no clearing: each
+    // residual pass writes every block's record before the reduction reads it.
+    hipLaunchKernelGGL(dev::k_fill_i, dim3(nblk(kNoAllocTicketInts)), dim3(kBlock), 0, ctx->stream, tickets,
+                       kNoAllocTicketInts, 0);
+    if (!check_launch(ctx, "k_fill_i(scratch tickets)")) return MFS_ERR_HIP;
+    NoAllocView view(ctx, part, tickets, hist_dev, nsteps);
     return step_iterations(ctx, nsteps);
 }
 
