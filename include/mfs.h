@@ -282,8 +282,8 @@ int mfs_batch_create(int precision, mfs_batch** out);
  * the defaults.  Results do not depend on them. */
 typedef struct mfs_batch_options {
     int gradient_stream; /* the batch step's CalculateGradient: 1 on the batch stream, fused with the next
-                          * residual pass (default); 2 or 4 on a second stream, started after the update or
-                          * after the next iteration's forward sweep */
+                          * residual pass; 2 or 4 on a second stream, started after the update or after the
+                          * next iteration's forward sweep.  Default 1 for MFS_FP64, 4 for MFS_MIXED. */
 } mfs_batch_options;
 int mfs_batch_create_ex(int precision, const mfs_batch_options* opt, mfs_batch** out);
 int mfs_batch_destroy(mfs_batch* b);
