@@ -220,7 +220,12 @@ def mode_fp64(args, cpu_dir: str | None = None) -> dict:
         if os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get("bytes_per_launch")
-        roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        # the sweeps move HBM bytes, but their time is set by the DAG depth
+        # (sweep levels x a dependency hop and a chunk's arithmetic), not by
+        # bandwidth
+        limiter = "dependency latency (sweep levels)" if dom.endswith("sweep") else "hbm"
+        roof = {"kernel": dom, "bound": "hbm", "limiter": limiter, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "alg_bytes_per_launch": int(b), "avg_launch_ms": round(ms / cnt, 4)}
     iter_b = alg_bytes(ITER_BYTES, nF, nC, nB, nN)
