@@ -168,10 +168,14 @@ typedef struct mfs_ctx_options {
     int sweep_waves;    /* waves per sweep workgroup: 4, 8 or 16 (default 4) */
     int sweep_by_tiles; /* 0: flat level-order schedule (default); 1: one workgroup per tile */
     int gradient_stream; /* mfs_step's CalculateGradient: 1 on the context stream, fused with the next
-                          * residual pass; 2, 3, 4 on a second stream of the context, started after the
-                          * update, after the next iteration's residual pass, or after its forward sweep
-                          * (nothing in an iteration reads the gradient).  Default 3 for MFS_FP64, 4 for
-                          * MFS_MIXED (the fastest on 6.0M cells). */
+                          * residual pass; 2, 3, 4, 5 on a second stream of the context, started after the
+                          * update, after the next iteration's residual pass, after its forward sweep, or
+                          * after the wide levels of its forward sweep, when the thin tail begins
+                          * (nothing in an iteration reads the gradient).  Default 5 (the fastest on 6.0M
+                          * cells in both precisions). */
+    int sweep_tail_grid; /* workgroups of the sweeps' thin-tail launch: each sweep of the flat schedule runs
+                          * its wide levels and the run of thin levels that ends its order (at most 64
+                          * chunks each) as two launches.  0: default (64); -1: one launch per sweep. */
 } mfs_ctx_options;
 int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p, int precision, const mfs_ctx_options* opt,
                       mfs_ctx** out);
@@ -255,6 +259,8 @@ typedef struct mfs_ctx_info {
     size_t device_bytes;   /* device memory held by the context */
     int tile_cells, sweep_waves, sweep_tiles, sweep_workgroups, sweep_by_tiles;
     int compute_units;     /* CUs of the context's device (the sweep grid is sized on them) */
+    int sweep_tail_chunks; /* chunks of the sweeps' thin-tail launch (0: one launch per sweep) */
+    int sweep_tail_workgroups;
 } mfs_ctx_info;
 int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
 /* Kernel time accounting: when enabled, every launch of the named kernel

@@ -134,11 +134,11 @@ class Solver:
     """Device context with the reference's kernel entry points."""
 
     def __init__(self, grid: Grid, params: _lib.MfsParams, precision: int = MFS_FP64, tile_cells: int = 0,
-                 sweep_waves: int = 0, sweep_by_tiles: int = 0, gradient_stream: int = 0):
+                 sweep_waves: int = 0, sweep_by_tiles: int = 0, gradient_stream: int = 0, sweep_tail_grid: int = 0):
         self._h = C.c_void_p()
         view = grid.view()
         self.params = params
-        opt = _lib.MfsCtxOptions(tile_cells, sweep_waves, sweep_by_tiles, gradient_stream)
+        opt = _lib.MfsCtxOptions(tile_cells, sweep_waves, sweep_by_tiles, gradient_stream, sweep_tail_grid)
         check(lib().mfs_ctx_create_ex(C.byref(view), C.byref(params), precision, C.byref(opt), C.byref(self._h)),
               "mfs_ctx_create_ex")
         self.grid = grid

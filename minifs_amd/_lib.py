@@ -59,12 +59,13 @@ class MfsCtxInfo(C.Structure):
     _fields_ = [("nTCell", C.c_int), ("nTFace", C.c_int), ("nBFace", C.c_int), ("nTNode", C.c_int),
                 ("precision", C.c_int), ("sweep_levels", C.c_int), ("device_bytes", C.c_size_t),
                 ("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_tiles", C.c_int),
-                ("sweep_workgroups", C.c_int), ("sweep_by_tiles", C.c_int), ("compute_units", C.c_int)]
+                ("sweep_workgroups", C.c_int), ("sweep_by_tiles", C.c_int), ("compute_units", C.c_int),
+                ("sweep_tail_chunks", C.c_int), ("sweep_tail_workgroups", C.c_int)]
 
 
 class MfsCtxOptions(C.Structure):
     _fields_ = [("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_by_tiles", C.c_int),
-                ("gradient_stream", C.c_int)]
+                ("gradient_stream", C.c_int), ("sweep_tail_grid", C.c_int)]
 
 
 class MfsBatchOptions(C.Structure):

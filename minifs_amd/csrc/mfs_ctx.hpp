@@ -92,20 +92,27 @@ struct mfs_ctx {
     double* sdn = nullptr;   // per slot: |(x_fc - x_cc).n| on this cell's side
     int* sflip = nullptr;    // per cell: slots whose face has this cell second (bits 0-7), face count (8-11)
     int ghost_refresh = 0;   // SetGhostVariables after each update in mfs_step
-    // mfs_step's gradient on a second stream (gradient_stream option 2):
+    // mfs_step's gradient on a second stream (gradient_stream options 2-5):
     // iteration k's node values and gradient run beside iteration k+1's
-    // residual pass and sweeps.  sdone: recorded on `stream` after each
-    // update; gdone[k & 1]: on `gstream` after iteration k's gradient, which
+    // residual pass and sweeps.  sdone: recorded on `stream` where the
+    // gradient starts; gdone[k & 1]: on `gstream` after iteration k's gradient, which
     // reads the state buffer that the backward sweep of iteration k+2
     // overwrites; gpend[j]: gdone[j] recorded and not yet waited on.
     hipStream_t gstream = nullptr;
     hipEvent_t sdone = nullptr, gdone[2] = {nullptr, nullptr};
     bool gpend[2] = {false, false};
-    int gradient_stream = 1;  // 1: on the solver stream (fused with the next residual pass), 2-4: second stream
+    int gradient_stream = 1;  // 1: on the solver stream (fused with the next residual pass), 2-5: second stream
     int cell0 = 0;  // device id of mesh cell 0
     int nchunk = 0, lpc = 4;
     int nlev = 0;
     int sweep_grid = 0;  // resident workgroups of the sweep launch
+    // The sweeps' thin tail: positions [tail_u, nchunk) of the flat order
+    // are the longest run of thin levels (at most kThinLevelChunks chunks
+    // each) that ends the order.  With tail_grid > 0 each sweep runs as two
+    // launches, its wide levels on sweep_grid workgroups and its tail on
+    // tail_grid, so that other work can fill the CUs the tail leaves idle.
+    int tail_u = 0;
+    int tail_grid = 0;
     int compute_units = 0;  // CUs of the device
     // reductions and history
     double* part = nullptr;  // nblk * 8: 5 squared-norm partials, dt min, dt max, spare

@@ -73,20 +73,31 @@ constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wa
 // sweep of 6M cells); 64 counters cost well under a tenth of a sweep.
 constexpr int kTicketGroups = 64;
 constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
+constexpr unsigned kClampBlocks = 1024;  // grid of the ratio_dtmax clamp (grid-stride over the cells)
 // Where mfs_step runs CalculateGradient: 1 on the solver stream, fused with
-// the next iteration's residual pass; 2, 3, 4 on a second stream, started
-// after the update, after the next residual pass, after the next forward
-// sweep.
-// Measured on 6.0M cells (ms per iteration, modes 1/2/3/4): FP64
-// 4.47/4.51/4.23/4.50, mixed precision 3.63/3.66/3.55/3.34.
-constexpr int kDefaultGradientStreamFP64 = 3;
-constexpr int kDefaultGradientStreamMixed = 4;
+// the next iteration's residual pass; 2, 3, 4, 5 on a second stream,
+// started after the update, after the next residual pass, after the next
+// forward sweep, after the next forward sweep's wide levels.
+// Measured on 6.0M cells, ms per iteration, with one launch per sweep
+// (modes 1/2/3/4): FP64 4.47/4.51/4.23/4.50, mixed precision
+// 3.63/3.66/3.55/3.34; with the thin tail launched apart (modes 1/3/4/5):
+// FP64 4.44/4.14/4.15/4.04, mixed 3.47 (3), 3.33 (4), 3.33 (5).  Mode 5
+// runs the node values and the gradient in the CUs that the two sweeps'
+// thin tails leave idle.
+constexpr int kDefaultGradientStreamFP64 = 5;
+constexpr int kDefaultGradientStreamMixed = 5;
 // The same for the batch step (1: fused with the next residual pass; 2, 4:
 // second stream after the update / after the next forward sweep).  64
 // seeded solves (21.9M cells), ms per batch iteration, modes 1/2/4: FP64
 // 14.20/14.59/14.52, mixed precision 11.73/12.00/11.47.
 constexpr int kDefaultBatchGradientStreamFP64 = 1;
 constexpr int kDefaultBatchGradientStreamMixed = 4;
+// The sweeps' thin tail runs as a launch of its own on this many
+// workgroups (a thin level has at most 64 chunks), when it holds at least
+// kMinTailChunks chunks.  6.0M cells, FP64, gradient mode 5, ms per
+// iteration for 32/64/128/256 workgroups: 4.08/4.04/4.05/4.05.
+constexpr int kDefaultTailGrid = 64;
+constexpr int kMinTailChunks = 256;
 
 #include "mfs_kernels.hpp"
 #include "mfs_sweep.hpp"
@@ -544,17 +555,31 @@ bool launch_reduce(mfs_ctx* ctx, int mode, int slot) {
 template <int MAXF>
 bool launch_diag_t(mfs_ctx* ctx, bool clamp_only) {
     Timed t(ctx, clamp_only ? "clamp" : "diag");
-    hipLaunchKernelGGL((dev::k_diag<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->cfo,
-                       ctx->cf, ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt, ctx->diag,
-                       clamp_only ? ctx->run : nullptr, ctx->P.ratio_dtmax, ctx->K, ctx->diag32);
+    if (clamp_only)  // usually off: a small grid that returns at once
+        hipLaunchKernelGGL((dev::k_clamp<MAXF>), dim3(std::min<unsigned>(ctx->nblk_cell, dev::kClampBlocks)), dim3(kBlock),
+                           0, ctx->stream, ctx->nC, ctx->cfo, ctx->cf, ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt,
+                           ctx->diag, ctx->run, ctx->P.ratio_dtmax, ctx->K, ctx->diag32);
+    else
+        hipLaunchKernelGGL((dev::k_diag<MAXF>), dim3(ctx->nblk_cell), dim3(kBlock), 0, ctx->stream, ctx->nC, ctx->cfo,
+                           ctx->cf, ctx->fgeo, ctx->q[ctx->cur], ctx->vol, ctx->dt, ctx->diag, nullptr,
+                           ctx->P.ratio_dtmax, ctx->K, ctx->diag32);
     return check_launch(ctx, "k_diag");
 }
 bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
     return ctx->maxf == 4 ? launch_diag_t<4>(ctx, clamp_only) : launch_diag_t<6>(ctx, clamp_only);
 }
 
+// Workgroups of the thin-tail launch of a split sweep.
+int tail_workgroups(const mfs_ctx* ctx) {
+    const int units = (ctx->nchunk - ctx->tail_u + ctx->sweep_waves - 1) / ctx->sweep_waves;
+    return std::max(1, std::min({ctx->tail_grid, ctx->sweep_grid, units}));
+}
+
+// The two sweeps and the update.  `after` is enqueued after the forward
+// sweep, or, with at_tail on a split sweep, after its wide levels, before
+// its thin tail.
 template <int LPC, typename R>
-bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after_forward) {
+bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_tail) {
     dev::SweepArgs<R> A;
     A.nC = ctx->nC;
     A.nchunks = ctx->nchunk;
@@ -586,44 +611,63 @@ bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after_forward) {
     const dim3 block(64 * ctx->sweep_waves);
     const int units = A.flat ? (A.nchunks + ctx->sweep_waves - 1) / ctx->sweep_waves : A.ntiles;
     const dim3 grid((unsigned)std::max(1, std::min(ctx->sweep_grid, units)));
-    A.ngroups = std::min(kTicketGroups, (int)grid.x);
-    int* const tk_f = ctx->tickets;
-    int* const tk_b = ctx->tickets + kTicketGroups * kTicketStride;
+    // A split sweep runs its wide levels on the resident grid and its thin
+    // tail on a small one: the forward sweep ends with the tail and the
+    // backward sweep starts with it.
+    const bool split = A.flat && ctx->tail_grid > 0 && ctx->tail_u < A.nchunks;
+    const int nt = A.nchunks - ctx->tail_u;  // steps of the tail
+    const dim3 tgrid((unsigned)(split ? tail_workgroups(ctx) : 1));
+    // Each launch takes its tickets from one counter set and zeroes the
+    // other for the next launch.  The launches of an iteration alternate
+    // between the two sets and are even in number, so every iteration's
+    // first launch finds set 0 zeroed.
+    int* const tk[2] = {ctx->tickets, ctx->tickets + kTicketGroups * kTicketStride};
+    int set = 0;
+    auto launch = [&](const void* fn, dim3 g, int ubeg, int uend, const char* what) {
+        A.ubeg = ubeg;
+        A.uend = uend;
+        A.ngroups = std::min(kTicketGroups, (int)g.x);
+        A.tickets = tk[set];
+        A.tickets_next = tk[1 - set];
+        set ^= 1;
+        void* args[] = {&A, &ctx->K};
+        return hip_ok(ctx, hipLaunchKernel(fn, g, block, args, 0, ctx->stream), what);
+    };
     SweepTurn turn(ctx->device, ctx->stream);
     if (turn.error()) return sweep_turn_failed(ctx, turn.error());
     {
         Timed t(ctx, "fwd_sweep");
-        A.tickets = tk_f;
-        A.tickets_next = tk_b;
-        void* args[] = {&A, &ctx->K};
-        if (!hip_ok(ctx,
-                    hipLaunchKernel((const void*)dev::k_sweep<true, LPC, R>, grid, block, args, 0, ctx->stream),
-                    "k_sweep<fwd>"))
+        const void* fwd = (const void*)dev::k_sweep<true, LPC, R>;
+        if (split) {
+            if (!launch(fwd, grid, 0, ctx->tail_u, "k_sweep<fwd>")) return false;
+            if (at_tail && after && !after()) return false;
+            if (!launch(fwd, tgrid, ctx->tail_u, A.nchunks, "k_sweep<fwd> (thin tail)")) return false;
+        } else if (!launch(fwd, grid, 0, A.nchunks, "k_sweep<fwd>")) {
             return false;
+        }
     }
-    if (after_forward && !after_forward()) return false;
+    if ((!at_tail || !split) && after && !after()) return false;
     {
         Timed t(ctx, "bwd_sweep");
-        A.tickets = tk_b;
-        A.tickets_next = tk_f;
-        void* args[] = {&A, &ctx->K};
-        if (!hip_ok(ctx,
-                    hipLaunchKernel((const void*)dev::k_sweep<false, LPC, R>, grid, block, args, 0,
-                                               ctx->stream),
-                    "k_sweep<bwd>"))
+        const void* bwd = (const void*)dev::k_sweep<false, LPC, R>;
+        if (split) {
+            if (!launch(bwd, tgrid, 0, nt, "k_sweep<bwd> (thin tail)") ||
+                !launch(bwd, grid, nt, A.nchunks, "k_sweep<bwd>"))
+                return false;
+        } else if (!launch(bwd, grid, 0, A.nchunks, "k_sweep<bwd>")) {
             return false;
+        }
     }
     if (!turn.done()) return sweep_turn_failed(ctx, turn.error());
     ctx->cur = 1 - ctx->cur;
     return true;
 }
-// after_forward, if set, is enqueued between the two sweeps.
-bool launch_sweeps(mfs_ctx* ctx, const std::function<bool()>& after_forward = nullptr) {
+bool launch_sweeps(mfs_ctx* ctx, const std::function<bool()>& after = nullptr, bool at_tail = false) {
     if (ctx->precision == MFS_MIXED)
-        return ctx->lpc == 4 ? launch_sweeps_t<4, float>(ctx, after_forward)
-                             : launch_sweeps_t<8, float>(ctx, after_forward);
-    return ctx->lpc == 4 ? launch_sweeps_t<4, double>(ctx, after_forward)
-                         : launch_sweeps_t<8, double>(ctx, after_forward);
+        return ctx->lpc == 4 ? launch_sweeps_t<4, float>(ctx, after, at_tail)
+                             : launch_sweeps_t<8, float>(ctx, after, at_tail);
+    return ctx->lpc == 4 ? launch_sweeps_t<4, double>(ctx, after, at_tail)
+                         : launch_sweeps_t<8, double>(ctx, after, at_tail);
 }
 
 // CalculateGradient: node values, then the gradient.  With `next` set (the
@@ -804,8 +848,13 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
         set_error("mfs_ctx_create: tile_cells must be >= 64 and sweep_waves 4, 8 or 16");
         return MFS_ERR_ARG;
     }
-    if (gradient_stream < 1 || gradient_stream > 4) {
-        set_error("mfs_ctx_create: gradient_stream must be 0 (default) or 1 to 4");
+    if (gradient_stream < 1 || gradient_stream > 5) {
+        set_error("mfs_ctx_create: gradient_stream must be 0 (default) or 1 to 5");
+        return MFS_ERR_ARG;
+    }
+    const int tail_grid = (opt && opt->sweep_tail_grid) ? opt->sweep_tail_grid : dev::kDefaultTailGrid;
+    if (tail_grid < -1) {
+        set_error("mfs_ctx_create: sweep_tail_grid must be -1, 0 (default) or a workgroup count");
         return MFS_ERR_ARG;
     }
     *out = nullptr;
@@ -855,6 +904,12 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     ctx->K.betq = 1.0 + ctx->K.alpq * ctx->K.phiq;
     ctx->nlev = L.nlev;
     ctx->nchunk = (int)(L.chunk.size() / 2);
+    // the sweeps' thin tail: the thin levels that end the flat order (a
+    // level's chunks are contiguous there and share the flag)
+    ctx->tail_u = ctx->nchunk;
+    while (ctx->tail_u > 0 && (L.crun[4 * (size_t)(ctx->tail_u - 1) + 1] & kThinChunkFlag)) --ctx->tail_u;
+    if (ctx->tail_u == 0 || ctx->nchunk - ctx->tail_u < dev::kMinTailChunks) ctx->tail_u = ctx->nchunk;
+    ctx->tail_grid = (tail_grid > 0 && !sweep_by_tiles && ctx->tail_u < ctx->nchunk) ? tail_grid : 0;
     ctx->ntiles = (int)L.tile_chunk.size() - 1;
     ctx->lpc = L.lpc;
     ctx->nblk_cell = (int)nblk_xcd(L.nC);
@@ -1137,9 +1192,10 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
     // The gradient feeds nothing in the iteration (CalculateGradient's
     // output is not read by the flux, time step or sweeps), so it may run
     // beside the next iteration; only the state buffer it reads must
-    // outlive it.  gradient_stream 2/3/4: iteration k's node values and
+    // outlive it.  gradient_stream 2/3/4/5: iteration k's node values and
     // gradient start on the second stream after iteration k's update /
-    // iteration k+1's residual pass / iteration k+1's forward sweep.
+    // iteration k+1's residual pass / iteration k+1's forward sweep / the
+    // wide levels of that sweep (before its thin tail).
     const int mode = ctx->gstream ? ctx->gradient_stream : 1;
     const bool side = mode >= 2;
     const bool fuse = !side && gradient_fuses_next(ctx);
@@ -1179,8 +1235,8 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
         }
         // ForwardLUSGS: sweeps + update
         std::function<bool()> after_fwd;
-        if (side && mode == 4 && grad_q) after_fwd = [&] { return side_gradient(ctx->sdone); };
-        if (!launch_sweeps(ctx, after_fwd)) return MFS_ERR_HIP;
+        if (side && mode >= 4 && grad_q) after_fwd = [&] { return side_gradient(ctx->sdone); };
+        if (!launch_sweeps(ctx, after_fwd, mode == 5)) return MFS_ERR_HIP;
         // SetGhostVariables, when enabled (main.cpp:283 has the call commented out)
         if (ctx->ghost_refresh && !launch_ghosts(ctx)) return MFS_ERR_HIP;
         // CalculateGradient (fused: with the next iteration's first pass)
@@ -1488,6 +1544,8 @@ extern "C" int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info) {
                                                                                  ctx->sweep_waves);
     info->sweep_by_tiles = ctx->sweep_by_tiles;
     info->compute_units = ctx->compute_units;
+    info->sweep_tail_chunks = ctx->tail_grid > 0 ? ctx->nchunk - ctx->tail_u : 0;
+    info->sweep_tail_workgroups = ctx->tail_grid > 0 ? tail_workgroups(ctx) : 0;
     return MFS_OK;
 }
 
@@ -1686,6 +1744,8 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     A.err = c->err;
     A.tickets = A.tickets_next = nullptr;  // the batch sweeps assign (solve, position) items instead
     A.ngroups = 0;
+    A.ubeg = 0;
+    A.uend = c->nchunk;
     if constexpr (sizeof(R) == sizeof(double)) {
         E.ell_w = c->ell_w;
         E.qn = c->qn;

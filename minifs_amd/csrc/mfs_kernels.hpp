@@ -586,6 +586,20 @@ __global__ __launch_bounds__(kBlock) void k_diag(int nC, const int* __restrict__
                     diag32);
 }
 
+// The ratio_dtmax clamp alone (compute_timestep.cpp:929-945) over a small
+// grid: every workgroup reads the reduction's flag first and returns when
+// the clamp is off, which is the rule at the reference's ratio_dtmax.
+template <int MAXF>
+__global__ __launch_bounds__(kBlock) void k_clamp(int nC, const int* __restrict__ cfo, const int* __restrict__ cf,
+                                                  const double4* __restrict__ fgeo, const double* __restrict__ q,
+                                                  const double* __restrict__ vol, double* __restrict__ dt,
+                                                  double* __restrict__ diag, const double* __restrict__ run,
+                                                  double ratio_dtmax, FlowConst K, float* __restrict__ diag32) {
+    if (run[2] == 0.0) return;
+    for (int c = (int)(blockIdx.x * kBlock + threadIdx.x); c < nC; c += (int)(gridDim.x * kBlock))
+        diag_body<MAXF>(c, nC, cfo, cf, fgeo, q, vol, dt, diag, run, ratio_dtmax, K, diag32);
+}
+
 // FP32 operator inputs of a mixed-precision context from the FP64 state and
 // Diag (interior cells; the ghost part of the snapshot never changes).
 __global__ __launch_bounds__(kBlock) void k_mixed_inputs(int nC, const double* __restrict__ q,
@@ -610,7 +624,30 @@ __device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__
     constexpr int NW = kReduceBlock / 64;
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double a = kBIGd, b = 0.0;
-    for (int k = threadIdx.x; k < nblk; k += kReduceBlock) {
+    // each thread folds records k, k+1024, ... in that order; the records
+    // of a group of kReduceBatch are loaded together before they are added,
+    // so the thread waits for one round trip per group, not per record
+    constexpr int kReduceBatch = 8;
+    int k = threadIdx.x;
+    for (; k + (kReduceBatch - 1) * kReduceBlock < nblk; k += kReduceBatch * kReduceBlock) {
+        double r[kReduceBatch][7];
+#pragma unroll
+        for (int i = 0; i < kReduceBatch; ++i)
+#pragma unroll
+            for (int j = 0; j < 7; ++j) r[i][j] = part[(size_t)(k + i * kReduceBlock) * 8 + j];
+#pragma unroll
+        for (int i = 0; i < kReduceBatch; ++i) {
+            if (mode & 1) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) s[j] += r[i][j];
+            }
+            if (mode & 2) {
+                a = smin(a, r[i][5]);
+                b = smax(b, r[i][6]);
+            }
+        }
+    }
+    for (; k < nblk; k += kReduceBlock) {
         const double* r = part + (size_t)k * 8;
         if (mode & 1) {
 #pragma unroll

@@ -121,6 +121,11 @@ struct SweepArgs {
     int* tickets;                           // this sweep's ticket counters (flat schedule)
     int* tickets_next;                      // the other direction's, zeroed here for its next sweep
     int ngroups;                            // ticket groups in use (<= kTicketGroups and <= workgroups)
+    // flat schedule: this launch walks sweep steps [ubeg, uend) of the
+    // sweep's order (step u is position u forward, nchunks-1-u backward);
+    // a sweep split in two launches runs its wide levels and its thin tail
+    // apart, each launch ending when all its steps are done
+    int ubeg, uend;
 };
 
 // Adds the LPC per-face products of this lane's cell in slot order (the
@@ -351,8 +356,8 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(SweepArgs<R> A, FlowConst
         if (lane == 0) tk = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tk = __shfl(tk, 0, 64);
         while (true) {
-            const int u = grp + ng * tk;
-            if (u >= A.nchunks) return;
+            const int u = A.ubeg + grp + ng * tk;
+            if (u >= A.uend) return;
             int next = 0;  // taken now, used after this chunk: its latency overlaps the chunk
             if (lane == 0) next = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (!sweep_chunk<FWD, LPC, R>(A, K, FWD ? u : A.nchunks - 1 - u)) return;

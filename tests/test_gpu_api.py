@@ -168,20 +168,70 @@ def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles)
     b.close()
 
 
-@pytest.mark.parametrize("cells,precision,ghosts,mode", [("tet", mfs.MFS_FP64, False, 2), ("tet", mfs.MFS_MIXED, False, 3),
-                                                          ("tet", mfs.MFS_FP64, True, 4), ("hex", mfs.MFS_FP64, False, 4),
-                                                          ("prism", mfs.MFS_MIXED, True, 2), ("tet", mfs.MFS_MIXED, False, 4)])
-def test_gradient_on_second_stream_same_results(cells, precision, ghosts, mode):
-    """mfs_step with the gradient on a second stream (started after the
-    update, after the next residual pass or after the next forward sweep)
-    gives the same bits as with the gradient on the solver stream: every
-    field and every history record, across several mfs_step calls and with
-    kernel entry points in between."""
-    print(f"[api] gradient stream {mode} vs 1: {cells} precision={precision} ghosts={ghosts}", flush=True)
+@pytest.mark.parametrize("precision,tail_grid,gradient_stream", [(mfs.MFS_FP64, 0, 1), (mfs.MFS_FP64, 1, 3),
+                                                                (mfs.MFS_FP64, 7, 5), (mfs.MFS_MIXED, 0, 4),
+                                                                (mfs.MFS_MIXED, 100000, 5)])
+def test_sweep_tail_launch_does_not_change_results(precision, tail_grid, gradient_stream):
+    """A sweep whose order ends in a run of thin levels runs them as a launch
+    of their own (the forward sweep's last launch, the backward sweep's
+    first).  The chunks, their order and their arithmetic are the same, so
+    every field and history record is the same bits as with one launch per
+    sweep, for tail grids of one workgroup, a few, the default and more than
+    the tail has chunks (clamped)."""
+    print(f"[api] sweep tail grid {tail_grid} precision={precision} gradient_stream={gradient_stream}", flush=True)
     p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
-    g = mfs.Grid.box_bump(10, cells=cells).decouple().preprocess(p)
+    g = mfs.Grid.box_bump(20).decouple().preprocess(p)
+    a = mfs.Solver(g, p, precision=precision, gradient_stream=gradient_stream, sweep_tail_grid=-1)
+    b = mfs.Solver(g, p, precision=precision, gradient_stream=gradient_stream, sweep_tail_grid=tail_grid)
+    ia, ib = a.info(), b.info()
+    assert ia["sweep_tail_chunks"] == 0 and ia["sweep_tail_workgroups"] == 0
+    assert ib["sweep_tail_chunks"] == 694  # 66 of the 85 levels
+    want = {0: 64, 1: 1, 7: 7}.get(tail_grid, (694 + 3) // 4)
+    assert ib["sweep_tail_workgroups"] == want
+    for s in (a, b):
+        s.step(5)
+        # one iteration through the entry points: forward_lusgs takes the
+        # same sweep launches as mfs_step
+        s.zero_residuals()
+        s.inviscid_flux()
+        s.compute_timestep()
+        s.forward_lusgs()
+        s.calculate_gradient()
+        s.iteration_done()
+        s.step(3)
+        s.sync()
+    for nm in FIELDS:
+        np.testing.assert_array_equal(a.field(nm), b.field(nm), err_msg=nm)
+    np.testing.assert_array_equal(a.history(), b.history())
+    a.close()
+    b.close()
+    g.close()
+
+
+@pytest.mark.parametrize("cells,precision,ghosts,mode,n", [("tet", mfs.MFS_FP64, False, 2, 10),
+                                                            ("tet", mfs.MFS_MIXED, False, 3, 10),
+                                                            ("tet", mfs.MFS_FP64, True, 4, 10),
+                                                            ("hex", mfs.MFS_FP64, False, 4, 10),
+                                                            ("prism", mfs.MFS_MIXED, True, 2, 10),
+                                                            ("tet", mfs.MFS_MIXED, False, 4, 10),
+                                                            ("tet", mfs.MFS_FP64, False, 5, 20),
+                                                            ("tet", mfs.MFS_MIXED, True, 5, 20),
+                                                            ("tet", mfs.MFS_FP64, False, 5, 10)])
+def test_gradient_on_second_stream_same_results(cells, precision, ghosts, mode, n):
+    """mfs_step with the gradient on a second stream (started after the
+    update, after the next residual pass, after the next forward sweep or
+    after that sweep's wide levels) gives the same bits as with the gradient
+    on the solver stream: every field and every history record, across
+    several mfs_step calls and with kernel entry points in between.  The
+    20^3 x 6 mesh has a thin tail of 694 chunks, so its sweeps run as two
+    launches each; the 10^3 x 6 mesh has none (mode 5 then starts the
+    gradient after the forward sweep)."""
+    print(f"[api] gradient stream {mode} vs 1: {cells} N={n} precision={precision} ghosts={ghosts}", flush=True)
+    p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
+    g = mfs.Grid.box_bump(n, cells=cells).decouple().preprocess(p)
     a = mfs.Solver(g, p, precision=precision, gradient_stream=1)
     b = mfs.Solver(g, p, precision=precision, gradient_stream=mode)
+    assert (b.info()["sweep_tail_chunks"] > 0) == (n == 20)
     for s in (a, b):
         s.ghost_refresh(ghosts)
         s.step(7)
