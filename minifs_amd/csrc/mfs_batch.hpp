@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_b_flux_ts(const BatchSolve<R>* __res
     const int lb = (int)blockIdx.x - S.blk0_cell;
     const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;  // XCD-aware inside the solve
     flux_ts_slot_body<MAXF == 4 ? 4 : 8, MIXED>(lg * kBlock + (int)threadIdx.x, S.nC, S.sw.snb, S.sgeo64, S.sdn,
-                                                S.sflip, S.q, S.vol, S.res, S.dt, S.diag, S.part + (size_t)lb * 8, S.K,
+                                                S.sflip, S.q, S.vol, S.res, S.dt, S.diag, S.part + lb, S.nblk_cell, S.K,
                                                 S.cfl, S.cfl_min, S.q32, S.diag32);
 }
 
@@ -151,6 +151,6 @@ __global__ __launch_bounds__(kBlock) void k_b_grad_flux_ts(const BatchSolve<R>* 
         grad_body<MAXF, R>(c, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn, S.btype, S.qnew, S.qn, S.vol,
                            S.gx, S.gy, S.gz);
     flux_ts_slot_body<MAXF == 4 ? 4 : 8, MIXED>(c, S1.nC, S1.sw.snb, S1.sgeo64, S1.sdn, S1.sflip, S1.q, S1.vol,
-                                                S1.res, S1.dt, S1.diag, S1.part + (size_t)lb * 8, S1.K, S1.cfl,
+                                                S1.res, S1.dt, S1.diag, S1.part + lb, S1.nblk_cell, S1.K, S1.cfl,
                                                 S1.cfl_min, S1.q32, S1.diag32);
 }

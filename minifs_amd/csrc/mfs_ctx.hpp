@@ -115,7 +115,7 @@ struct mfs_ctx {
     int tail_grid = 0;
     int compute_units = 0;  // CUs of the device
     // reductions and history
-    double* part = nullptr;  // nblk * 8: 5 squared-norm partials, dt min, dt max, spare
+    double* part = nullptr;  // 8 * nblk, value-major: 5 squared-norm partials, dt min, dt max, spare
     double* run = nullptr;   // [0] dt_max [1] dt_min [2] clamp flag [3..7] last norms
     int* err = nullptr;      // [0] sweep wait limit hit
     double* hist = nullptr;

@@ -330,18 +330,24 @@ bool check_launch(mfs_ctx* ctx, const char* what) { return hip_ok(ctx, hipGetLas
 // waves the other keeps from starting, until their bounded waits end both
 // sweeps with an error.  So every sweep pair (forward and backward, of a
 // context or of a batch, from any host thread) is ordered after the
-// previous one launched on the same device: under a per-device lock, the
-// launching stream waits on the GPU, through an event, for the device's
-// last sweep pair, and records that event after its own.  Kernels that do
-// not wait on other waves (everything but the sweeps) may still overlap.
+// previous one launched on the same device: under a per-device lock, a
+// stream that is not the one that launched the device's last sweep pair
+// records an event on that stream (which then follows its sweeps, enqueued
+// under the same lock) and waits on it on the GPU.  A stream that sweeps
+// after itself needs no event: each event record costs the stream about
+// 7 us of idle GPU between two kernels.  Kernels that do not wait on other
+// waves (everything but the sweeps) may still overlap.
 // The table is allocated once and never freed: static destructors run at
 // process exit, after which the HIP runtime may be gone.
 constexpr int kMaxDevices = 64;
 struct SweepOrder {
     std::mutex mu;
-    hipEvent_t last = nullptr;          // recorded after the device's latest sweep pair
-    hipStream_t last_stream = nullptr;  // the stream it was recorded on
+    hipEvent_t last = nullptr;          // recorded on last_stream when another stream takes its turn
+    hipStream_t last_stream = nullptr;  // stream of the device's latest sweep pair (null: none yet)
 };
+// last_stream of a table whose latest sweeping stream was destroyed: `last`
+// was recorded on that stream before, and follows its sweeps.
+const hipStream_t kGoneStream = reinterpret_cast<hipStream_t>(~uintptr_t(0));
 SweepOrder* const g_sweep_order = new SweepOrder[kMaxDevices];
 
 // Creates the device's ordering event, once; called with `device` current
@@ -363,8 +369,8 @@ bool sweep_order_setup(int device) {
     return true;
 }
 
-// Holds the device's lock from before the sweeps are enqueued until the
-// event is recorded after them.
+// Holds the device's lock from before the sweeps are enqueued until they
+// are.
 class SweepTurn {
   public:
     SweepTurn(int device, hipStream_t stream) : o_(g_sweep_order[device & (kMaxDevices - 1)]), st_(stream), lk_(o_.mu) {
@@ -376,16 +382,15 @@ class SweepTurn {
             err_ = "sweep order event of the device not created";
             return;
         }
-        if (o_.last_stream && o_.last_stream != st_ && hipStreamWaitEvent(st_, o_.last, 0) != hipSuccess)
+        if (!o_.last_stream || o_.last_stream == st_) return;
+        if (o_.last_stream != kGoneStream && hipEventRecord(o_.last, o_.last_stream) != hipSuccess)
+            err_ = "hipEventRecord(sweep order)";
+        else if (hipStreamWaitEvent(st_, o_.last, 0) != hipSuccess)
             err_ = "hipStreamWaitEvent(sweep order)";
     }
     // after the sweeps are enqueued
     bool done() {
         if (err_) return false;
-        if (hipEventRecord(o_.last, st_) != hipSuccess) {
-            err_ = "hipEventRecord(sweep order)";
-            return false;
-        }
         o_.last_stream = st_;
         return true;
     }
@@ -410,7 +415,9 @@ void forget_sweep_stream(int device, hipStream_t stream) {
     if (device < 0 || device >= kMaxDevices) return;
     SweepOrder& o = g_sweep_order[device];
     std::lock_guard<std::mutex> lk(o.mu);
-    if (o.last_stream == stream) o.last_stream = reinterpret_cast<hipStream_t>(~uintptr_t(0));
+    if (o.last_stream != stream) return;
+    (void)hipEventRecord(o.last, stream);  // follows the stream's sweeps
+    o.last_stream = kGoneStream;
 }
 
 // ---------------------------------------------------------------------------
@@ -425,8 +432,10 @@ void forget_sweep_stream(int device, hipStream_t stream) {
 // g_noalloc_scratch; only that address is passed to kernels.
 static_assert(kNoAllocPartDoubles ==
                   (size_t)(((kNoAllocMaxCells + dev::kBlock - 1) / dev::kBlock + dev::kXcd - 1) / dev::kXcd * dev::kXcd) * 8,
-              "scratch partials: one 8-double record per cell block of the largest context");
+              "scratch partials: 8 doubles per cell block of the largest context");
 static_assert(kNoAllocTicketInts == 2 * (size_t)kTicketGroups * kTicketStride, "scratch tickets: both sweeps");
+static_assert(7 * (kNoAllocPartDoubles / 8) + 7 * (size_t)dev::kReduceBlock <= kNoAllocPartDoubles,
+              "scratch partials: room for the reduction's folds after the largest context's records");
 
 std::mutex g_noalloc_mu;
 void* g_noalloc_scratch[kMaxDevices] = {};
@@ -545,9 +554,28 @@ bool launch_mixed_inputs(mfs_ctx* ctx) {
     return check_launch(ctx, "k_mixed_inputs");
 }
 
+// Above this many cell blocks the reduction runs as k_reduce_fold +
+// k_reduce_tree (the same bits as k_reduce, from many CUs).
+constexpr int kFoldMinBlocks = 2048;
+// The folds of k_reduce_fold go to the partials' buffer from its spare
+// row (index 7) on: 7 * kReduceBlock doubles.
+size_t partial_doubles(int nblk) {
+    return std::max<size_t>(8 * (size_t)nblk, 7 * (size_t)nblk + 7 * (size_t)dev::kReduceBlock);
+}
+
 bool launch_reduce(mfs_ctx* ctx, int mode, int slot) {
     Timed t(ctx, "reduce");
-    hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(dev::kReduceBlock), 0, ctx->stream, (int)ctx->nblk_cell, ctx->part, ctx->run,
+    const int nb = (int)ctx->nblk_cell;
+    if (nb > kFoldMinBlocks) {
+        double* const fold = ctx->part + (size_t)7 * nb;
+        hipLaunchKernelGGL(dev::k_reduce_fold, dim3(dev::kReduceBlock * dev::kFoldLanes / dev::kFoldThreads),
+                           dim3(dev::kFoldThreads), 0, ctx->stream, nb, ctx->part, fold);
+        if (!check_launch(ctx, "k_reduce_fold")) return false;
+        hipLaunchKernelGGL(dev::k_reduce_tree, dim3(1), dim3(dev::kReduceBlock), 0, ctx->stream, fold, ctx->run,
+                           ctx->hist, slot, mode, ctx->P.ratio_dtmax);
+        return check_launch(ctx, "k_reduce_tree");
+    }
+    hipLaunchKernelGGL(dev::k_reduce, dim3(1), dim3(dev::kReduceBlock), 0, ctx->stream, nb, ctx->part, ctx->run,
                        ctx->hist, slot, mode, ctx->P.ratio_dtmax);
     return check_launch(ctx, "k_reduce");
 }
@@ -981,7 +1009,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
              dalloc(ctx, ctx->ell_w32, ell_w.size(), "ell_w32") && dalloc(ctx, ctx->fgeo32, L.nF, "fgeo32") &&
              dalloc(ctx, ctx->sgeo32, L.cfs.size(), "slot geometry (FP32)");
     }
-    ok = ok && dalloc(ctx, ctx->part, 8 * (size_t)ctx->nblk_cell, "partials") && dalloc(ctx, ctx->run, 8, "run") &&
+    ok = ok && dalloc(ctx, ctx->part, partial_doubles((int)ctx->nblk_cell), "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err") &&
          dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets") &&
          dalloc(ctx, ctx->gface, (size_t)L.nB, "ghost faces");
@@ -1227,7 +1255,10 @@ int step_iterations(mfs_ctx* ctx, int nsteps) {
         if (!launch_reduce(ctx, 1 | 2 | 4, ctx->hist_n)) return MFS_ERR_HIP;
         if (!launch_diag(ctx, true)) return MFS_ERR_HIP;  // acts only when the ratio_dtmax clamp triggers
         // The backward sweep writes the state buffer that the gradient of
-        // two iterations back reads.
+        // two iterations back reads.  (Waiting between the forward sweep's
+        // two launches instead, beside the event that starts the next
+        // gradient, let the node values reach the CUs before the thin tail
+        // and slowed the tail by 90 us.)
         const int par = ctx->iter_done & 1;
         if (side && ctx->gpend[par]) {
             if (!hip_ok(ctx, hipStreamWaitEvent(ctx->stream, ctx->gdone[par], 0), "wait gradient")) return MFS_ERR_HIP;
@@ -1312,7 +1343,7 @@ extern "C" int mfs_step_noalloc(mfs_ctx* ctx, int nsteps, double* hist_dev) {
         set_error("mfs_step_noalloc: the context belongs to a batch; step it with mfs_batch_step");
         return MFS_ERR_STATE;
     }
-    if (ctx->nC > kNoAllocMaxCells || (size_t)ctx->nblk_cell * 8 > kNoAllocPartDoubles) {
+    if (ctx->nC > kNoAllocMaxCells || partial_doubles((int)ctx->nblk_cell) > kNoAllocPartDoubles) {
         set_error("mfs_step_noalloc: " + std::to_string(ctx->nC) + " cells exceed the static scratch's limit of " +
                   std::to_string(kNoAllocMaxCells) + " cells (use mfs_step)");
         return MFS_ERR_ARG;

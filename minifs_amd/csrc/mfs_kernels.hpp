@@ -32,7 +32,7 @@ struct Real<float> {
 };
 
 template <int NV, int NW = kBlock / 64>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* out, int stride = 1) {
     __shared__ double s[NW][NV];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1)
@@ -48,7 +48,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
         for (int j = 0; j < NV; ++j) {
             double a = s[0][j];
             for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a += s[k][j];
-            out[j] = a;
+            out[(size_t)j * stride] = a;
         }
     }
 }
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(int nC, const int* __restrict__
     double s2[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
-    block_sum<5>(s2, part + (size_t)blockIdx.x * 8);
+    block_sum<5>(s2, part + blockIdx.x, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_timestep(int nC, const int* __restri
             diag[c] = dg;
         }
     }
-    block_minmax(vmin, vmax, part + (size_t)blockIdx.x * 8 + 5, part + (size_t)blockIdx.x * 8 + 6);
+    block_minmax(vmin, vmax, part + (size_t)5 * gridDim.x + blockIdx.x, part + (size_t)6 * gridDim.x + blockIdx.x);
 }
 
 // Residual, local time step and LU-SGS diagonal in one pass over a cell's
@@ -433,7 +433,7 @@ __device__ __forceinline__ void flux_ts_body(int c, int nC, const int* __restric
                                              const double2* __restrict__ fdn, const double* __restrict__ q,
                                              const double* __restrict__ vol, double* __restrict__ res,
                                              double* __restrict__ dt, double* __restrict__ diag,
-                                             double* __restrict__ pblk, const FlowConst& K, double cfl,
+                                             double* __restrict__ pblk, int pstride, const FlowConst& K, double cfl,
                                              double cfl_min, float* __restrict__ q32, float* __restrict__ diag32) {
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double vmin = kBIGd, vmax = 0.0;
@@ -464,8 +464,8 @@ __device__ __forceinline__ void flux_ts_body(int c, int nC, const int* __restric
     double s2[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
-    block_sum<5>(s2, pblk);
-    block_minmax(vmin, vmax, pblk + 5, pblk + 6);
+    block_sum<5>(s2, pblk, pstride);
+    block_minmax(vmin, vmax, pblk + (size_t)5 * pstride, pblk + (size_t)6 * pstride);
 }
 
 // The same from the per-slot tables laid out by cell (neighbour, normal
@@ -479,7 +479,7 @@ __device__ __forceinline__ void flux_ts_slot_body(int c, int nC, const int* __re
                                                   const int* __restrict__ sflip, const double* __restrict__ q,
                                                   const double* __restrict__ vol, double* __restrict__ res,
                                                   double* __restrict__ dt, double* __restrict__ diag,
-                                                  double* __restrict__ pblk, const FlowConst& K, double cfl,
+                                                  double* __restrict__ pblk, int pstride, const FlowConst& K, double cfl,
                                                   double cfl_min, float* __restrict__ q32,
                                                   float* __restrict__ diag32) {
     double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -511,8 +511,8 @@ __device__ __forceinline__ void flux_ts_slot_body(int c, int nC, const int* __re
     double s2[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) s2[j] = r[j] * r[j];
-    block_sum<5>(s2, pblk);
-    block_minmax(vmin, vmax, pblk + 5, pblk + 6);
+    block_sum<5>(s2, pblk, pstride);
+    block_minmax(vmin, vmax, pblk + (size_t)5 * pstride, pblk + (size_t)6 * pstride);
 }
 
 template <int LPC, bool MIXED>
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_ts_slot(int nC, const int* __re
                                                          FlowConst K, double cfl, double cfl_min,
                                                          float* __restrict__ q32, float* __restrict__ diag32) {
     flux_ts_slot_body<LPC, MIXED>(xcd_block() * kBlock + (int)threadIdx.x, nC, snb, sgeo, sdn, sflip, q, vol, res,
-                                  dt, diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
+                                  dt, diag, part + blockIdx.x, (int)gridDim.x, K, cfl, cfl_min, q32, diag32);
 }
 
 template <int MAXF, bool MIXED>
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_ts(int nC, const int* __restric
                                                     double cfl_min, float* __restrict__ q32,
                                                     float* __restrict__ diag32) {
     flux_ts_body<MAXF, MIXED>(xcd_block() * kBlock + (int)threadIdx.x, nC, cfo, cf, fcell, fgeo, fdn, q, vol, res, dt,
-                              diag, part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
+                              diag, part + blockIdx.x, (int)gridDim.x, K, cfl, cfl_min, q32, diag32);
 }
 
 // Diag alone (forward_lusgs.cpp:94-99 + CalDiagLUSGS :554-595), from dt.
@@ -617,47 +617,20 @@ __global__ __launch_bounds__(kBlock) void k_mixed_inputs(int nC, const double* _
 // (running max/min of compute_timestep.cpp:914-918, clamp flag), 4 = write
 // history record `slot` from run[].
 // One workgroup of kReduceBlock threads reads each block's partials once
-// (norms and extrema together) and sums them in a fixed tree.
+// (norms and extrema together) and sums them in a fixed tree.  The partials
+// are stored value-major, part[j * nblk + block] for the five squared
+// norms (j = 0..4), dt min (5) and dt max (6), so that every load of a wave
+// reads 64 consecutive doubles: with one 64-byte record per block, a wave's
+// load touched 32 cache lines, and a 6.0M-cell reduction took 57 us in the
+// single workgroup's load unit.
 constexpr int kReduceBlock = 1024;
-__device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__ part, double* __restrict__ run,
-                                            double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
+
+// The second half of a reduction: the block tree over the threads' folds
+// (s: squared-norm sums, a: dt min, b: dt max), the running extrema, the
+// clamp flag and the history record.
+__device__ __forceinline__ void reduce_finish(double (&s)[5], double a, double b, double* __restrict__ run,
+                                              double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
     constexpr int NW = kReduceBlock / 64;
-    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double a = kBIGd, b = 0.0;
-    // each thread folds records k, k+1024, ... in that order; the records
-    // of a group of kReduceBatch are loaded together before they are added,
-    // so the thread waits for one round trip per group, not per record
-    constexpr int kReduceBatch = 8;
-    int k = threadIdx.x;
-    for (; k + (kReduceBatch - 1) * kReduceBlock < nblk; k += kReduceBatch * kReduceBlock) {
-        double r[kReduceBatch][7];
-#pragma unroll
-        for (int i = 0; i < kReduceBatch; ++i)
-#pragma unroll
-            for (int j = 0; j < 7; ++j) r[i][j] = part[(size_t)(k + i * kReduceBlock) * 8 + j];
-#pragma unroll
-        for (int i = 0; i < kReduceBatch; ++i) {
-            if (mode & 1) {
-#pragma unroll
-                for (int j = 0; j < 5; ++j) s[j] += r[i][j];
-            }
-            if (mode & 2) {
-                a = smin(a, r[i][5]);
-                b = smax(b, r[i][6]);
-            }
-        }
-    }
-    for (; k < nblk; k += kReduceBlock) {
-        const double* r = part + (size_t)k * 8;
-        if (mode & 1) {
-#pragma unroll
-            for (int j = 0; j < 5; ++j) s[j] += r[j];
-        }
-        if (mode & 2) {
-            a = smin(a, r[5]);
-            b = smax(b, r[6]);
-        }
-    }
     if (mode & 1) {
         __shared__ double tot[5];
         block_sum<5, NW>(s, tot);
@@ -685,9 +658,120 @@ __device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__
         h[6] = run[1];
     }
 }
+
+__device__ __forceinline__ void reduce_body(int nblk, const double* __restrict__ part, double* __restrict__ run,
+                                            double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double a = kBIGd, b = 0.0;
+    // each thread folds records k, k+1024, ... in that order; the records
+    // of a group of kReduceBatch are loaded together before they are added
+    // (a group past the end loads a valid record in place of the missing
+    // ones and skips them), so the thread waits for one round trip per
+    // group, not per record
+    constexpr int kReduceBatch = 7;  // 8 spills registers
+    for (int k = threadIdx.x; k < nblk; k += kReduceBatch * kReduceBlock) {
+        double r[kReduceBatch][7];
+#pragma unroll
+        for (int i = 0; i < kReduceBatch; ++i) {
+            const int e = min(k + i * kReduceBlock, nblk - 1);
+#pragma unroll
+            for (int j = 0; j < 7; ++j) r[i][j] = part[(size_t)j * nblk + e];
+        }
+#pragma unroll
+        for (int i = 0; i < kReduceBatch; ++i) {
+            if (k + i * kReduceBlock >= nblk) break;
+            if (mode & 1) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) s[j] += r[i][j];
+            }
+            if (mode & 2) {
+                a = smin(a, r[i][5]);
+                b = smax(b, r[i][6]);
+            }
+        }
+    }
+    reduce_finish(s, a, b, run, hist, slot, mode, ratio_dtmax);
+}
 __global__ __launch_bounds__(kReduceBlock) void k_reduce(int nblk, const double* __restrict__ part, double* __restrict__ run,
                                                    double* __restrict__ hist, int slot, int mode, double ratio_dtmax) {
     reduce_body(nblk, part, run, hist, slot, mode, ratio_dtmax);
+}
+
+// The same reduction in two launches, for many blocks.  One workgroup
+// reading all the records is bound by the misses one CU keeps in flight
+// (42 us for the 23,438 records of 6.0M cells, whatever the batching).
+// k_reduce_fold does the threads' folds on many CUs: the fold of thread t
+// of k_reduce (records t, t+1024, ... in that order) is shared by
+// kFoldLanes consecutive lanes, each loading up to kFoldRecs of its
+// records at once; the running values pass from lane to lane in record
+// order, so every addition is k_reduce's, in k_reduce's order.  The folds
+// go to fold[j * 1024 + t] (j: 5 sums, dt min, dt max) and k_reduce_tree
+// finishes them exactly as k_reduce does, so both give the same bits.
+constexpr int kFoldLanes = 8, kFoldRecs = 4;
+constexpr int kFoldThreads = 256;
+__global__ __launch_bounds__(kFoldThreads) void k_reduce_fold(int nblk, const double* __restrict__ part,
+                                                              double* __restrict__ fold) {
+    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int t = g / kFoldLanes;  // thread of k_reduce whose fold this is
+    const int sub = g % kFoldLanes;
+    const int lane = (int)threadIdx.x & 63;
+    const int src = lane - sub + (sub + kFoldLanes - 1) % kFoldLanes;  // the previous lane of this fold
+    const int cnt = t < nblk ? (nblk - 1 - t) / kReduceBlock + 1 : 0;  // records of thread t
+    const int maxcnt = (nblk + kReduceBlock - 1) / kReduceBlock;       // the same bound for every lane
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double a = kBIGd, b = 0.0;
+    for (int base = 0; base < maxcnt; base += kFoldLanes * kFoldRecs) {
+        double r[kFoldRecs][7];
+#pragma unroll
+        for (int m = 0; m < kFoldRecs; ++m) {
+            const int i = base + sub * kFoldRecs + m;
+            const int e = min(t + i * kReduceBlock, nblk - 1);
+#pragma unroll
+            for (int j = 0; j < 7; ++j) r[m][j] = part[(size_t)j * nblk + e];
+        }
+        // lane `step` adds its records to the running values, which then
+        // move on to the next lane (from the last lane back to the first,
+        // which starts the next round)
+        for (int step = 0; step < kFoldLanes; ++step) {
+            if (sub == step) {
+#pragma unroll
+                for (int m = 0; m < kFoldRecs; ++m) {
+                    if (base + sub * kFoldRecs + m >= cnt) break;
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) s[j] += r[m][j];
+                    a = smin(a, r[m][5]);
+                    b = smax(b, r[m][6]);
+                }
+            }
+            double ts[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) ts[j] = __shfl(s[j], src, 64);
+            const double ta = __shfl(a, src, 64), tb = __shfl(b, src, 64);
+            if (sub == (step + 1) % kFoldLanes) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) s[j] = ts[j];
+                a = ta;
+                b = tb;
+            }
+        }
+    }
+    if (sub == 0 && t < kReduceBlock) {  // the complete fold is back at the first lane
+#pragma unroll
+        for (int j = 0; j < 5; ++j) fold[j * kReduceBlock + t] = s[j];
+        fold[5 * kReduceBlock + t] = a;
+        fold[6 * kReduceBlock + t] = b;
+    }
+}
+__global__ __launch_bounds__(kReduceBlock) void k_reduce_tree(const double* __restrict__ fold, double* __restrict__ run,
+                                                        double* __restrict__ hist, int slot, int mode,
+                                                        double ratio_dtmax) {
+    const int t = (int)threadIdx.x;
+    double s[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s[j] = (mode & 1) ? fold[j * kReduceBlock + t] : 0.0;
+    const double a = (mode & 2) ? fold[5 * kReduceBlock + t] : kBIGd;
+    const double b = (mode & 2) ? fold[6 * kReduceBlock + t] : 0.0;
+    reduce_finish(s, a, b, run, hist, slot, mode, ratio_dtmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -973,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_flux_ts(int nC, int n, const in
     const int c = xcd_block() * kBlock + (int)threadIdx.x;
     grad_tet_body<R>(c, nC, n, snb, sgeo_r, snode, depmask, q, qn, vol, gx, gy, gz);  // no-op past nC
     flux_ts_slot_body<4, MIXED>(c, nC, snb, sgeo, sdn, sflip, q, vol, res, dt, diag,
-                                part + (size_t)blockIdx.x * 8, K, cfl, cfl_min, q32, diag32);
+                                part + blockIdx.x, (int)gridDim.x, K, cfl, cfl_min, q32, diag32);
 }
 
 template <int MAXF, typename R>

@@ -197,11 +197,14 @@ def test_batch_with_symmetry_and_interface_patches(tmp_path, precision):
 
 @pytest.mark.parametrize("precision", [mfs.MFS_FP64, mfs.MFS_MIXED])
 def test_batch_of_large_solves_equals_standalone(precision):
-    """Solves of 94k, 162k and 384k cells (thousands of chunks each, long
-    interleaved item lists, many XCD groups per solve): the same bits as
-    alone after a few fused iterations."""
+    """Solves of 94k, 162k, 384k and 584k cells (thousands of chunks each,
+    long interleaved item lists, many XCD groups per solve): the same bits
+    as alone after a few fused iterations.  Alone, the 584k-cell solve
+    (2,282 cell blocks) reduces its residual norms in two launches, the
+    threads' folds on many CUs and then the tree; in the batch one workgroup
+    does both: the history records are the same bits."""
     print(f"[batch] large solves precision={precision}", flush=True)
-    cases = [_case(25, 0.5, 0.0, 61), _case(30, 0.8, 1.25, 62), _case(40, 0.4, -2.0, 63)]
+    cases = [_case(25, 0.5, 0.0, 61), _case(30, 0.8, 1.25, 62), _case(40, 0.4, -2.0, 63), _case(46, 0.7, 2.0, 64)]
     b = mfs.Batch(precision)
     members = [b.add(g, p) for g, p in cases]
     b.step(3)
