@@ -264,7 +264,10 @@ typedef struct mfs_ctx_info {
 } mfs_ctx_info;
 int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
 /* Kernel time accounting: when enabled, every launch of the named kernel
- * group is bracketed with HIP events on the context stream. */
+ * group is bracketed with HIP events on the stream it runs on.  enable: 0
+ * off, 1 every kernel group, 2 the two sweeps only ("fwd_sweep",
+ * "bwd_sweep"; each event record leaves the stream idle for a few us
+ * between two kernels, so fewer events disturb the iteration less). */
 int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
 /* names: "flux","timestep","flux_ts" (both fused, mfs_step),"reduce","diag","clamp",
  * "fwd_sweep","bwd_sweep","ghosts","node","gradient","grad_flux_ts" (mfs_step on

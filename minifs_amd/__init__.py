@@ -221,8 +221,9 @@ class Solver:
         check(lib().mfs_ctx_get_info(self._h, C.byref(i)), "mfs_ctx_get_info")
         return {f: getattr(i, f) for f, _ in i._fields_}
 
-    def enable_timing(self, on: bool = True):
-        check(lib().mfs_ctx_enable_timing(self._h, 1 if on else 0), "mfs_ctx_enable_timing")
+    def enable_timing(self, on: bool | int = True):
+        """True/1: time every kernel group; 2: the two sweeps only; False/0: off."""
+        check(lib().mfs_ctx_enable_timing(self._h, int(on)), "mfs_ctx_enable_timing")
 
     def timing(self, name: str) -> tuple[float, int]:
         ms = C.c_double()

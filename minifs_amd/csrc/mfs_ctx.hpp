@@ -127,7 +127,7 @@ struct mfs_ctx {
     double cfl_min = 0.0;
 
     // timing
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every kernel group, 2 the sweeps only
     std::map<std::string, mfs::KernelTimer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;

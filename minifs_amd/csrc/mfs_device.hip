@@ -280,20 +280,28 @@ inline unsigned nblk(size_t n, int b = kBlock) { return (unsigned)((n + b - 1) /
 inline unsigned nblk_xcd(size_t n) { return (nblk(n) + dev::kXcd - 1) / dev::kXcd * dev::kXcd; }
 
 // Event bracketing of one launch group when timing is enabled.
+// With `attach`, the caller passes a and b to its first and last launch
+// (hipExtLaunchKernel), which set them to the kernels' own start and end:
+// no event packet goes into the stream.  Otherwise a and b are recorded on
+// the stream before and after the launches of the scope.
 struct Timed {
     mfs_ctx* ctx;
     const char* name;
     hipStream_t st;
+    bool attach;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(mfs_ctx* c, const char* nm, hipStream_t s = nullptr) : ctx(c), name(nm), st(s ? s : c->stream) {
+    Timed(mfs_ctx* c, const char* nm, hipStream_t s = nullptr, bool att = false)
+        : ctx(c), name(nm), st(s ? s : c->stream), attach(att) {
         if (!ctx->timing) return;
+        if (ctx->timing == 2 && std::strcmp(nm, "fwd_sweep") != 0 && std::strcmp(nm, "bwd_sweep") != 0) return;
         a = take();
         b = take();
-        if (a) (void)hipEventRecord(a, st);
+        if (!a || !b) a = b = nullptr;
+        if (a && !attach) (void)hipEventRecord(a, st);
     }
     ~Timed() {
         if (!ctx->timing || !a || !b) return;
-        (void)hipEventRecord(b, st);
+        if (!attach) (void)hipEventRecord(b, st);
         ctx->pending.push_back({name, {a, b}});
     }
     hipEvent_t take() {
@@ -651,7 +659,9 @@ bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_t
     // first launch finds set 0 zeroed.
     int* const tk[2] = {ctx->tickets, ctx->tickets + kTicketGroups * kTicketStride};
     int set = 0;
-    auto launch = [&](const void* fn, dim3 g, int ubeg, int uend, const char* what) {
+    // ev0 / ev1: the timing events that this launch's start / end set
+    auto launch = [&](const void* fn, dim3 g, int ubeg, int uend, const char* what, hipEvent_t ev0,
+                      hipEvent_t ev1) {
         A.ubeg = ubeg;
         A.uend = uend;
         A.ngroups = std::min(kTicketGroups, (int)g.x);
@@ -659,30 +669,30 @@ bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_t
         A.tickets_next = tk[1 - set];
         set ^= 1;
         void* args[] = {&A, &ctx->K};
-        return hip_ok(ctx, hipLaunchKernel(fn, g, block, args, 0, ctx->stream), what);
+        return hip_ok(ctx, hipExtLaunchKernel(fn, g, block, args, 0, ctx->stream, ev0, ev1, 0), what);
     };
     SweepTurn turn(ctx->device, ctx->stream);
     if (turn.error()) return sweep_turn_failed(ctx, turn.error());
     {
-        Timed t(ctx, "fwd_sweep");
+        Timed t(ctx, "fwd_sweep", nullptr, true);
         const void* fwd = (const void*)dev::k_sweep<true, LPC, R>;
         if (split) {
-            if (!launch(fwd, grid, 0, ctx->tail_u, "k_sweep<fwd>")) return false;
+            if (!launch(fwd, grid, 0, ctx->tail_u, "k_sweep<fwd>", t.a, nullptr)) return false;
             if (at_tail && after && !after()) return false;
-            if (!launch(fwd, tgrid, ctx->tail_u, A.nchunks, "k_sweep<fwd> (thin tail)")) return false;
-        } else if (!launch(fwd, grid, 0, A.nchunks, "k_sweep<fwd>")) {
+            if (!launch(fwd, tgrid, ctx->tail_u, A.nchunks, "k_sweep<fwd> (thin tail)", nullptr, t.b)) return false;
+        } else if (!launch(fwd, grid, 0, A.nchunks, "k_sweep<fwd>", t.a, t.b)) {
             return false;
         }
     }
     if ((!at_tail || !split) && after && !after()) return false;
     {
-        Timed t(ctx, "bwd_sweep");
+        Timed t(ctx, "bwd_sweep", nullptr, true);
         const void* bwd = (const void*)dev::k_sweep<false, LPC, R>;
         if (split) {
-            if (!launch(bwd, tgrid, 0, nt, "k_sweep<bwd> (thin tail)") ||
-                !launch(bwd, grid, nt, A.nchunks, "k_sweep<bwd>"))
+            if (!launch(bwd, tgrid, 0, nt, "k_sweep<bwd> (thin tail)", t.a, nullptr) ||
+                !launch(bwd, grid, nt, A.nchunks, "k_sweep<bwd>", nullptr, t.b))
                 return false;
-        } else if (!launch(bwd, grid, 0, A.nchunks, "k_sweep<bwd>")) {
+        } else if (!launch(bwd, grid, 0, A.nchunks, "k_sweep<bwd>", t.a, t.b)) {
             return false;
         }
     }
@@ -1310,7 +1320,7 @@ struct NoAllocView {
         c->hist = h;
         c->hist_n = 0;
         c->hist_cap = n;
-        c->timing = false;
+        c->timing = 0;
     }
     ~NoAllocView() {
         ctx->part = part;
@@ -1582,7 +1592,7 @@ extern "C" int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info) {
 
 extern "C" int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable) {
     if (!ctx) return MFS_ERR_ARG;
-    ctx->timing = enable != 0;
+    ctx->timing = enable == 2 ? 2 : (enable ? 1 : 0);
     return MFS_OK;
 }
 

@@ -208,6 +208,33 @@ def test_sweep_tail_launch_does_not_change_results(precision, tail_grid, gradien
     g.close()
 
 
+def test_kernel_timing_levels():
+    """Timing level 1 brackets every kernel group with HIP events, level 2
+    the two sweeps only; neither changes a result bit."""
+    print("[api] kernel timing levels", flush=True)
+    p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
+    g = mfs.Grid.box_bump(20).decouple().preprocess(p)
+    ref = mfs.Solver(g, p)
+    ref.step(4)
+    for level in (1, 2):
+        s = mfs.Solver(g, p)
+        s.enable_timing(level)
+        s.step(4)
+        s.sync()
+        for nm in ("fwd_sweep", "bwd_sweep"):
+            ms, n = s.timing(nm)
+            assert n == 4 and ms > 0, (level, nm, ms, n)
+        for nm in ("flux_ts", "reduce", "node", "gradient"):
+            ms, n = s.timing(nm)
+            assert (n > 0) == (level == 1), (level, nm, n)
+        for nm in FIELDS:
+            np.testing.assert_array_equal(s.field(nm), ref.field(nm), err_msg=nm)
+        np.testing.assert_array_equal(s.history(), ref.history())
+        s.close()
+    ref.close()
+    g.close()
+
+
 @pytest.mark.parametrize("cells,precision,ghosts,mode,n", [("tet", mfs.MFS_FP64, False, 2, 10),
                                                             ("tet", mfs.MFS_MIXED, False, 3, 10),
                                                             ("tet", mfs.MFS_FP64, True, 4, 10),
