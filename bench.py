@@ -70,6 +70,10 @@ KERNEL_BYTES = {
 }
 # kernels that mfs_step runs on the context's second stream by default
 SIDE_KERNELS = ("node", "gradient")
+# a sweep run as two launches (wide levels, thin tail) is timed as a whole
+# and per launch; each launch's algorithmic bytes are its cells' share of
+# the sweep's
+SWEEP_LAUNCHES = {"fwd_sweep": ("fwd_sweep_wide", "fwd_sweep_tail"), "bwd_sweep": ("bwd_sweep_tail", "bwd_sweep_wide")}
 # iterations after the timed region with every kernel group timed
 KERNEL_TABLE_STEPS = 10
 # whole iteration, plain FP64 (268 nF + 840 nC + 356 nB + 92 nN)
@@ -190,7 +194,7 @@ def timed_steps(s, steps: int, warmup: int, kernel_timing: int):
     s.sync()
     elapsed = time.perf_counter() - t1
     kern = {}
-    for k in KERNEL_BYTES:
+    for k in list(KERNEL_BYTES) + [part for parts in SWEEP_LAUNCHES.values() for part in parts]:
         ms, cnt = s.timing(k)
         if cnt:
             kern[k] = (ms, cnt)
@@ -219,16 +223,44 @@ def mode_fp64(args, cpu_dir: str | None = None) -> dict:
     ms_per_step = elapsed / args.steps * 1e3
     value = nC * args.steps / elapsed / 1e6
     hist = s.history()
+    info_ctx = s.info()
+    tail_cells = info_ctx["sweep_tail_cells"]
+
+    def launch_bytes(k):
+        """Algorithmic bytes of one launch of kernel group k."""
+        for sweep, parts in SWEEP_LAUNCHES.items():
+            if k in parts:
+                cells = tail_cells if k.endswith("_tail") else nC - tail_cells
+                return alg_bytes(KERNEL_BYTES[sweep], nF, nC, nB, nN) * cells / nC
+        return alg_bytes(KERNEL_BYTES[k], nF, nC, nB, nN)
+
     roof = None
+    sweeps = None
     if kern:
-        # the dominant kernel of the critical path: the solver stream's
-        # kernels (node values and the gradient run on the context's second
-        # stream beside them, so their event times overlap the sweeps')
-        dom = max((k for k in kern if k not in SIDE_KERNELS), key=lambda k: kern[k][0])
+        # the dominant kernel launch of the critical path: the solver
+        # stream's launches (node values and the gradient run on the
+        # context's second stream beside them); a split sweep counts as its
+        # two launches
+        split = [k for k in SWEEP_LAUNCHES if SWEEP_LAUNCHES[k][0] in kern]
+        launches = [k for k in kern if k not in SIDE_KERNELS and k not in split]
+        dom = max(launches, key=lambda k: kern[k][0])
         ms, cnt = kern[dom]
         avg_s = ms / cnt / 1e3
-        b = alg_bytes(KERNEL_BYTES[dom], nF, nC, nB, nN)
+        b = launch_bytes(dom)
         achieved = b / avg_s / 1e9
+        sweeps = {}
+        for k in SWEEP_LAUNCHES:
+            if k not in kern:
+                continue
+            sb = alg_bytes(KERNEL_BYTES[k], nF, nC, nB, nN)
+            span = kern[k][0] / kern[k][1]
+            sweeps[k] = {"span_ms": round(span, 4), "alg_bytes": int(sb),
+                         "achieved_GBs": round(sb / (span / 1e3) / 1e9, 2)}
+            for part in SWEEP_LAUNCHES[k]:
+                if part in kern:
+                    pm = kern[part][0] / kern[part][1]
+                    sweeps[k][part + "_ms"] = round(pm, 4)
+                    sweeps[k][part + "_achieved_GBs"] = round(launch_bytes(part) / (pm / 1e3) / 1e9, 2)
         traffic = None
         tf = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
         if os.path.exists(tf):
@@ -237,13 +269,17 @@ def mode_fp64(args, cpu_dir: str | None = None) -> dict:
         # the sweeps move HBM bytes, but their time is set by the DAG depth
         # (sweep levels x a dependency hop and a chunk's arithmetic), not by
         # bandwidth
-        limiter = "dependency latency (sweep levels)" if dom.endswith("sweep") else "hbm"
+        limiter = ("dependency waits and scattered gathers (wide sweep levels)" if dom.endswith("_wide") else
+                   "dependency latency (thin sweep levels)" if dom.endswith("_tail") else
+                   "dependency latency (sweep levels)" if dom.endswith("sweep") else "hbm")
         roof = {"kernel": dom, "bound": "hbm", "limiter": limiter, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "alg_bytes_per_launch": int(b), "avg_launch_ms": round(ms / cnt, 4)}
+        if dom.endswith("_wide"):
+            roof["launch"] = (f"the sweep's wide levels: {nC - tail_cells} of its {nC} cells; its last "
+                              f"{tail_cells} cells (the thin tail) run as a second launch")
     iter_b = alg_bytes(ITER_BYTES, nF, nC, nB, nN)
-    info_ctx = s.info()
     s.close()
     # the same iteration with the gradient on the solver stream (no second
     # stream beside the sweeps): each kernel's own duration, and what the
@@ -255,7 +291,7 @@ def mode_fp64(args, cpu_dir: str | None = None) -> dict:
         s1.close()
         dom = roof["kernel"]
         ms1, cnt1 = kern1[dom]
-        b = alg_bytes(KERNEL_BYTES[dom], nF, nC, nB, nN)
+        b = launch_bytes(dom)
         a1 = b / (ms1 / cnt1 / 1e3) / 1e9
         isolated = {"gradient_stream": 1, "ms_per_step": round(el1 / args.steps * 1e3, 4),
                     "value": round(nC * args.steps / el1 / 1e6, 3),
@@ -309,6 +345,7 @@ def mode_fp64(args, cpu_dir: str | None = None) -> dict:
         "iteration_roofline": {"alg_bytes_per_iter": int(iter_b),
                                "achieved_GBs": round(iter_b / (ms_per_step / 1e3) / 1e9, 2),
                                "frac": round(iter_b / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)},
+        "sweeps": sweeps,
         "kernel_ms_per_step": {k: round(v[0] / v[1], 4) for k, v in kern_all.items()},
         "kernel_table": f"HIP events around every kernel group, {KERNEL_TABLE_STEPS} iterations after the timed "
                         "region (whose events bracket the sweeps only)",

@@ -260,18 +260,23 @@ typedef struct mfs_ctx_info {
     int tile_cells, sweep_waves, sweep_tiles, sweep_workgroups, sweep_by_tiles;
     int compute_units;     /* CUs of the context's device (the sweep grid is sized on them) */
     int sweep_tail_chunks; /* chunks of the sweeps' thin-tail launch (0: one launch per sweep) */
+    int sweep_tail_cells;  /* their cells */
     int sweep_tail_workgroups;
 } mfs_ctx_info;
 int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info);
 /* Kernel time accounting: when enabled, every launch of the named kernel
  * group is bracketed with HIP events on the stream it runs on.  enable: 0
- * off, 1 every kernel group, 2 the two sweeps only ("fwd_sweep",
- * "bwd_sweep"; each event record leaves the stream idle for a few us
- * between two kernels, so fewer events disturb the iteration less). */
+ * off, 1 every kernel group, 2 the two sweeps only.  The sweeps' events are
+ * attached to their launches (they cost the stream nothing); the others
+ * are event records, each of which leaves the stream idle for a few us
+ * between two kernels. */
 int mfs_ctx_enable_timing(mfs_ctx* ctx, int enable);
 /* names: "flux","timestep","flux_ts" (both fused, mfs_step),"reduce","diag","clamp",
- * "fwd_sweep","bwd_sweep","ghosts","node","gradient","grad_flux_ts" (mfs_step on
- * tetrahedra: a gradient with the next iteration's flux_ts);
+ * "fwd_sweep","bwd_sweep" (a sweep from its first launch's start to its last
+ * launch's end), "fwd_sweep_wide","fwd_sweep_tail","bwd_sweep_tail",
+ * "bwd_sweep_wide" (each launch of a sweep run as two, wide levels and thin
+ * tail),"ghosts","node","gradient","grad_flux_ts" (mfs_step on tetrahedra: a
+ * gradient with the next iteration's flux_ts);
  * returns accumulated milliseconds and launch count. */
 int mfs_ctx_get_timing(mfs_ctx* ctx, const char* name, double* ms, int* launches);
 int mfs_ctx_reset_timing(mfs_ctx* ctx);

@@ -60,7 +60,7 @@ class MfsCtxInfo(C.Structure):
                 ("precision", C.c_int), ("sweep_levels", C.c_int), ("device_bytes", C.c_size_t),
                 ("tile_cells", C.c_int), ("sweep_waves", C.c_int), ("sweep_tiles", C.c_int),
                 ("sweep_workgroups", C.c_int), ("sweep_by_tiles", C.c_int), ("compute_units", C.c_int),
-                ("sweep_tail_chunks", C.c_int), ("sweep_tail_workgroups", C.c_int)]
+                ("sweep_tail_chunks", C.c_int), ("sweep_tail_cells", C.c_int), ("sweep_tail_workgroups", C.c_int)]
 
 
 class MfsCtxOptions(C.Structure):

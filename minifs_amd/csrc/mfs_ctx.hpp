@@ -113,6 +113,7 @@ struct mfs_ctx {
     // tail_grid, so that other work can fill the CUs the tail leaves idle.
     int tail_u = 0;
     int tail_grid = 0;
+    int tail_cells = 0;  // cells of the chunks at positions [tail_u, nchunk)
     int compute_units = 0;  // CUs of the device
     // reductions and history
     double* part = nullptr;  // 8 * nblk, value-major: 5 squared-norm partials, dt min, dt max, spare
@@ -129,6 +130,7 @@ struct mfs_ctx {
     // timing
     int timing = 0;  // 0 off, 1 every kernel group, 2 the sweeps only
     std::map<std::string, mfs::KernelTimer> timers;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;  // timed intervals
+    std::vector<hipEvent_t> pending_events;  // the events they use, each once (intervals may share one)
     std::vector<hipEvent_t> event_pool;
 };

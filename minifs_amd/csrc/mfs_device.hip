@@ -280,29 +280,26 @@ inline unsigned nblk(size_t n, int b = kBlock) { return (unsigned)((n + b - 1) /
 inline unsigned nblk_xcd(size_t n) { return (nblk(n) + dev::kXcd - 1) / dev::kXcd * dev::kXcd; }
 
 // Event bracketing of one launch group when timing is enabled.
-// With `attach`, the caller passes a and b to its first and last launch
-// (hipExtLaunchKernel), which set them to the kernels' own start and end:
-// no event packet goes into the stream.  Otherwise a and b are recorded on
-// the stream before and after the launches of the scope.
+// Events recorded on the stream before and after the launches of a scope
+// (timing level 1; the sweeps time themselves, SweepTimes below).
 struct Timed {
     mfs_ctx* ctx;
     const char* name;
     hipStream_t st;
-    bool attach;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(mfs_ctx* c, const char* nm, hipStream_t s = nullptr, bool att = false)
-        : ctx(c), name(nm), st(s ? s : c->stream), attach(att) {
-        if (!ctx->timing) return;
-        if (ctx->timing == 2 && std::strcmp(nm, "fwd_sweep") != 0 && std::strcmp(nm, "bwd_sweep") != 0) return;
+    Timed(mfs_ctx* c, const char* nm, hipStream_t s = nullptr) : ctx(c), name(nm), st(s ? s : c->stream) {
+        if (ctx->timing != 1) return;
         a = take();
         b = take();
         if (!a || !b) a = b = nullptr;
-        if (a && !attach) (void)hipEventRecord(a, st);
+        if (a) (void)hipEventRecord(a, st);
     }
     ~Timed() {
-        if (!ctx->timing || !a || !b) return;
-        if (!attach) (void)hipEventRecord(b, st);
+        if (ctx->timing != 1 || !a || !b) return;
+        (void)hipEventRecord(b, st);
         ctx->pending.push_back({name, {a, b}});
+        ctx->pending_events.push_back(a);
+        ctx->pending_events.push_back(b);
     }
     hipEvent_t take() {
         if (!ctx->event_pool.empty()) {
@@ -324,11 +321,57 @@ void collect_timing(mfs_ctx* ctx) {
             t.ms += ms;
             t.launches += 1;
         }
-        ctx->event_pool.push_back(pe.second.first);
-        ctx->event_pool.push_back(pe.second.second);
     }
     ctx->pending.clear();
+    ctx->event_pool.insert(ctx->event_pool.end(), ctx->pending_events.begin(), ctx->pending_events.end());
+    ctx->pending_events.clear();
 }
+
+// The timing of one sweep: events attached to its launches (hipExtLaunchKernel),
+// e[0], e[1] around its first launch and, when it is split, e[2], e[3]
+// around its second.  It times the sweep from its first launch's start to
+// its last launch's end and, split, each launch on its own.
+struct SweepTimes {
+    mfs_ctx* ctx;
+    int n = 0;
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    SweepTimes(mfs_ctx* c, bool split) : ctx(c) {
+        if (!ctx->timing) return;
+        const int want = split ? 4 : 2;
+        for (int i = 0; i < want; ++i) {
+            hipEvent_t ev = nullptr;
+            if (!ctx->event_pool.empty()) {
+                ev = ctx->event_pool.back();
+                ctx->event_pool.pop_back();
+            } else if (hipEventCreate(&ev) != hipSuccess) {
+                ev = nullptr;
+            }
+            if (!ev) break;
+            e[n++] = ev;
+        }
+        if (n != want) {  // no timing rather than a partial one
+            for (int i = 0; i < n; ++i) ctx->event_pool.push_back(e[i]);
+            n = 0;
+            e[0] = e[1] = e[2] = e[3] = nullptr;
+        }
+    }
+    hipEvent_t start(int launch) const { return n ? e[2 * launch] : nullptr; }
+    hipEvent_t stop(int launch) const { return n ? e[2 * launch + 1] : nullptr; }
+    // after the launches: the names of the whole sweep and of its two launches
+    void done(const char* sweep, const char* first, const char* second) {
+        if (!n) return;
+        ctx->pending.push_back({sweep, {e[0], e[n - 1]}});
+        if (n == 4) {
+            ctx->pending.push_back({first, {e[0], e[1]}});
+            ctx->pending.push_back({second, {e[2], e[3]}});
+        }
+        for (int i = 0; i < n; ++i) ctx->pending_events.push_back(e[i]);
+        n = 0;
+    }
+    ~SweepTimes() {  // an error path left the events unused
+        for (int i = 0; i < n; ++i) ctx->event_pool.push_back(e[i]);
+    }
+};
 
 bool check_launch(mfs_ctx* ctx, const char* what) { return hip_ok(ctx, hipGetLastError(), what); }
 
@@ -674,27 +717,30 @@ bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_t
     SweepTurn turn(ctx->device, ctx->stream);
     if (turn.error()) return sweep_turn_failed(ctx, turn.error());
     {
-        Timed t(ctx, "fwd_sweep", nullptr, true);
+        SweepTimes t(ctx, split);
         const void* fwd = (const void*)dev::k_sweep<true, LPC, R>;
         if (split) {
-            if (!launch(fwd, grid, 0, ctx->tail_u, "k_sweep<fwd>", t.a, nullptr)) return false;
+            if (!launch(fwd, grid, 0, ctx->tail_u, "k_sweep<fwd>", t.start(0), t.stop(0))) return false;
             if (at_tail && after && !after()) return false;
-            if (!launch(fwd, tgrid, ctx->tail_u, A.nchunks, "k_sweep<fwd> (thin tail)", nullptr, t.b)) return false;
-        } else if (!launch(fwd, grid, 0, A.nchunks, "k_sweep<fwd>", t.a, t.b)) {
+            if (!launch(fwd, tgrid, ctx->tail_u, A.nchunks, "k_sweep<fwd> (thin tail)", t.start(1), t.stop(1)))
+                return false;
+        } else if (!launch(fwd, grid, 0, A.nchunks, "k_sweep<fwd>", t.start(0), t.stop(0))) {
             return false;
         }
+        t.done("fwd_sweep", "fwd_sweep_wide", "fwd_sweep_tail");
     }
     if ((!at_tail || !split) && after && !after()) return false;
     {
-        Timed t(ctx, "bwd_sweep", nullptr, true);
+        SweepTimes t(ctx, split);
         const void* bwd = (const void*)dev::k_sweep<false, LPC, R>;
         if (split) {
-            if (!launch(bwd, tgrid, 0, nt, "k_sweep<bwd> (thin tail)", t.a, nullptr) ||
-                !launch(bwd, grid, nt, A.nchunks, "k_sweep<bwd>", nullptr, t.b))
+            if (!launch(bwd, tgrid, 0, nt, "k_sweep<bwd> (thin tail)", t.start(0), t.stop(0)) ||
+                !launch(bwd, grid, nt, A.nchunks, "k_sweep<bwd>", t.start(1), t.stop(1)))
                 return false;
-        } else if (!launch(bwd, grid, 0, A.nchunks, "k_sweep<bwd>", t.a, t.b)) {
+        } else if (!launch(bwd, grid, 0, A.nchunks, "k_sweep<bwd>", t.start(0), t.stop(0))) {
             return false;
         }
+        t.done("bwd_sweep", "bwd_sweep_tail", "bwd_sweep_wide");
     }
     if (!turn.done()) return sweep_turn_failed(ctx, turn.error());
     ctx->cur = 1 - ctx->cur;
@@ -828,11 +874,9 @@ void free_all(mfs_ctx* ctx) {
     ctx->bufs.clear();
     if (ctx->hist) (void)hipFree(ctx->hist);
     ctx->hist = nullptr;
-    for (auto& pe : ctx->pending) {
-        (void)hipEventDestroy(pe.second.first);
-        (void)hipEventDestroy(pe.second.second);
-    }
+    for (auto e : ctx->pending_events) (void)hipEventDestroy(e);
     ctx->pending.clear();
+    ctx->pending_events.clear();
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     ctx->event_pool.clear();
     if (ctx->stream && ctx->owns_stream) {
@@ -947,6 +991,8 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     ctx->tail_u = ctx->nchunk;
     while (ctx->tail_u > 0 && (L.crun[4 * (size_t)(ctx->tail_u - 1) + 1] & kThinChunkFlag)) --ctx->tail_u;
     if (ctx->tail_u == 0 || ctx->nchunk - ctx->tail_u < dev::kMinTailChunks) ctx->tail_u = ctx->nchunk;
+    ctx->tail_cells = 0;
+    for (int u = ctx->tail_u; u < ctx->nchunk; ++u) ctx->tail_cells += L.crun[4 * (size_t)u + 1] & 0xff;
     ctx->tail_grid = (tail_grid > 0 && !sweep_by_tiles && ctx->tail_u < ctx->nchunk) ? tail_grid : 0;
     ctx->ntiles = (int)L.tile_chunk.size() - 1;
     ctx->lpc = L.lpc;
@@ -1586,6 +1632,7 @@ extern "C" int mfs_ctx_get_info(mfs_ctx* ctx, mfs_ctx_info* info) {
     info->sweep_by_tiles = ctx->sweep_by_tiles;
     info->compute_units = ctx->compute_units;
     info->sweep_tail_chunks = ctx->tail_grid > 0 ? ctx->nchunk - ctx->tail_u : 0;
+    info->sweep_tail_cells = ctx->tail_grid > 0 ? ctx->tail_cells : 0;
     info->sweep_tail_workgroups = ctx->tail_grid > 0 ? tail_workgroups(ctx) : 0;
     return MFS_OK;
 }

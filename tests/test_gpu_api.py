@@ -168,25 +168,29 @@ def test_layout_options_do_not_change_results(tile_cells, sweep_waves, by_tiles)
     b.close()
 
 
-@pytest.mark.parametrize("precision,tail_grid,gradient_stream", [(mfs.MFS_FP64, 0, 1), (mfs.MFS_FP64, 1, 3),
-                                                                (mfs.MFS_FP64, 7, 5), (mfs.MFS_MIXED, 0, 4),
-                                                                (mfs.MFS_MIXED, 100000, 5)])
-def test_sweep_tail_launch_does_not_change_results(precision, tail_grid, gradient_stream):
+@pytest.mark.parametrize("cells,precision,tail_grid,gradient_stream,tail_chunks",
+                         [("tet", mfs.MFS_FP64, 0, 1, 694), ("tet", mfs.MFS_FP64, 1, 3, 694),
+                          ("tet", mfs.MFS_FP64, 7, 5, 694), ("tet", mfs.MFS_MIXED, 0, 4, 694),
+                          ("tet", mfs.MFS_MIXED, 100000, 5, 694), ("hex", mfs.MFS_FP64, 0, 5, 472),
+                          ("prism", mfs.MFS_MIXED, 0, 5, 791)])
+def test_sweep_tail_launch_does_not_change_results(cells, precision, tail_grid, gradient_stream, tail_chunks):
     """A sweep whose order ends in a run of thin levels runs them as a launch
     of their own (the forward sweep's last launch, the backward sweep's
     first).  The chunks, their order and their arithmetic are the same, so
     every field and history record is the same bits as with one launch per
     sweep, for tail grids of one workgroup, a few, the default and more than
-    the tail has chunks (clamped)."""
-    print(f"[api] sweep tail grid {tail_grid} precision={precision} gradient_stream={gradient_stream}", flush=True)
+    the tail has chunks (clamped), on 20^3-hex boxes of tetrahedra (4 lanes
+    per cell), hexahedra and prisms (8 lanes per cell)."""
+    print(f"[api] sweep tail grid {tail_grid} {cells} precision={precision} gradient_stream={gradient_stream}",
+          flush=True)
     p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
-    g = mfs.Grid.box_bump(20).decouple().preprocess(p)
+    g = mfs.Grid.box_bump(20, cells=cells).decouple().preprocess(p)
     a = mfs.Solver(g, p, precision=precision, gradient_stream=gradient_stream, sweep_tail_grid=-1)
     b = mfs.Solver(g, p, precision=precision, gradient_stream=gradient_stream, sweep_tail_grid=tail_grid)
     ia, ib = a.info(), b.info()
     assert ia["sweep_tail_chunks"] == 0 and ia["sweep_tail_workgroups"] == 0
-    assert ib["sweep_tail_chunks"] == 694  # 66 of the 85 levels
-    want = {0: 64, 1: 1, 7: 7}.get(tail_grid, (694 + 3) // 4)
+    assert ib["sweep_tail_chunks"] == tail_chunks
+    want = {0: 64, 1: 1, 7: 7}.get(tail_grid, (tail_chunks + 3) // 4)
     assert ib["sweep_tail_workgroups"] == want
     for s in (a, b):
         s.step(5)
@@ -210,7 +214,9 @@ def test_sweep_tail_launch_does_not_change_results(precision, tail_grid, gradien
 
 def test_kernel_timing_levels():
     """Timing level 1 brackets every kernel group with HIP events, level 2
-    the two sweeps only; neither changes a result bit."""
+    the two sweeps only; neither changes a result bit.  The 20^3 x 6 mesh's
+    sweeps run as two launches each: each launch is timed, and a sweep's
+    span holds both."""
     print("[api] kernel timing levels", flush=True)
     p = mfs.make_params(mach=0.8, alpha=1.25, cfl=10.0)
     g = mfs.Grid.box_bump(20).decouple().preprocess(p)
@@ -221,9 +227,12 @@ def test_kernel_timing_levels():
         s.enable_timing(level)
         s.step(4)
         s.sync()
-        for nm in ("fwd_sweep", "bwd_sweep"):
-            ms, n = s.timing(nm)
-            assert n == 4 and ms > 0, (level, nm, ms, n)
+        for sweep, first, second in (("fwd_sweep", "fwd_sweep_wide", "fwd_sweep_tail"),
+                                     ("bwd_sweep", "bwd_sweep_tail", "bwd_sweep_wide")):
+            span = s.timing(sweep)
+            a, b = s.timing(first), s.timing(second)
+            assert span[1] == a[1] == b[1] == 4 and a[0] > 0 and b[0] > 0, (level, sweep, span, a, b)
+            assert span[0] >= a[0] + b[0], (level, sweep, span, a, b)
         for nm in ("flux_ts", "reduce", "node", "gradient"):
             ms, n = s.timing(nm)
             assert (n > 0) == (level == 1), (level, nm, n)
