@@ -162,6 +162,16 @@ __device__ __forceinline__ void load5r(const R* __restrict__ p, R v[5]) {
     for (int j = 0; j < 5; ++j) v[j] = p[j];
 }
 
+// This wave's staging area for a chunk's records (at most 80 words: 16
+// cells of 5 values), in LDS.  Its lanes write their cells' values, then
+// read back consecutive words; the wave barriers keep the compiler from
+// moving the accesses across each other (a wave's LDS accesses complete in
+// order).
+__device__ __forceinline__ double* stage_words() {
+    __shared__ double stage[kSweepBlock / 64][80];
+    return stage[threadIdx.x / 64];
+}
+
 // The part of the face term that does not depend on the neighbour's DQ:
 // the reference's FluxLUSGS3D split in two (FP64), or its increment form
 // (FP32 operator of mixed precision).
@@ -306,17 +316,38 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
         for (int j = 0; j < 5; ++j) D[j] -= acc[j] / tmp;
         dq_limit<R>(qc, D, (R)K.gamm1, (R)K.rho_min, (R)K.rho_max, (R)K.p_min, (R)K.p_max);
     }
-    if (active) {
+    // Stores.  The chunk's cells are contiguous, so its 5-value records
+    // form one run of words: lane w stores word w (and w + 64), and every
+    // store instruction writes whole, consecutive words.  Stored value by
+    // value from the cell's lanes, a record's words went out as 8- and
+    // 32-byte pieces of lines: a quarter of the sweeps' write requests were
+    // 32-byte partial writes, and the wide levels are bound by memory
+    // traffic.  On thin levels the values are published straight from the
+    // cell's lanes: there the staging round trip would lengthen the
+    // level's dependent chain.
+    const int nw = 5 * (rec.y & 0xff);  // words of the chunk's records
+    {
         R* const own = FWD ? A.dqf : A.dqb;    // published
         R* const other = FWD ? A.dqb : A.dqf;  // re-armed for the next sweep
-        const R sent = __builtin_bit_cast(R, kSent);
+        if (thin) {
+            if (active) {
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            if (j % LPC == s) {
-                st_dev<R>(own + (size_t)c * 5 + j, D[j]);
-                other[(size_t)c * 5 + j] = sent;
+                for (int j = 0; j < 5; ++j)
+                    if (j % LPC == s) st_dev<R>(own + (size_t)c * 5 + j, D[j]);
             }
+        } else {
+            double* const st = stage_words();
+            if (active) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    if (j % LPC == s) st[k * 5 + j] = (double)D[j];  // exact for R = float too
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int w = lane; w < nw; w += 64) st_dev<R>(own + (size_t)rec.x * 5 + w, (R)st[w]);
+            __builtin_amdgcn_wave_barrier();
         }
+        const R sent = __builtin_bit_cast(R, kSent);
+        for (int w = lane; w < nw; w += 64) other[(size_t)rec.x * 5 + w] = sent;
     }
     if (!FWD) {
         double qnew[5], dd[5];
@@ -330,10 +361,22 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
 #pragma unroll
         for (int j = 0; j < 5; ++j) dd[j] = (double)D[j];
         update_cell(qnew, dd, K);
-        if (active) {
+        if (thin) {
+            if (active) {
 #pragma unroll
-            for (int j = 0; j < 5; ++j)
-                if (j % LPC == s) A.qout[(size_t)c * 5 + j] = qnew[j];
+                for (int j = 0; j < 5; ++j)
+                    if (j % LPC == s) A.qout[(size_t)c * 5 + j] = qnew[j];
+            }
+        } else {
+            double* const st = stage_words();
+            if (active) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                    if (j % LPC == s) st[k * 5 + j] = qnew[j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int w = lane; w < nw; w += 64) A.qout[(size_t)rec.x * 5 + w] = st[w];
+            __builtin_amdgcn_wave_barrier();
         }
     }
     return true;
