@@ -661,6 +661,7 @@ template <int LPC, typename R>
 bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_tail) {
     dev::SweepArgs<R> A;
     A.nC = ctx->nC;
+    A.nq = ctx->nC + ctx->nB;
     A.nchunks = ctx->nchunk;
     A.ntiles = ctx->ntiles;
     A.tile_chunk = ctx->tile_chunk;
@@ -1818,6 +1819,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     auto& A = E.sw;
     A.nC = c->nC;
     A.nchunks = c->nchunk;
+    A.nq = c->nC + c->nB;
     A.ntiles = c->ntiles;
     A.tile_chunk = c->tile_chunk;
     A.pos = c->pos;

@@ -91,6 +91,54 @@ __device__ __forceinline__ float cell_bcast(float v) {
     }
 }
 
+// 5-value records read with 128-bit buffer loads: 3 loads for a 40-byte
+// record, 2 for a 20-byte one, where per-value loads take 5.  The sweeps
+// read their dependencies' published DQ this way: those loads bypass the
+// CU's cache, so each of them is a request to the memory system (the
+// plain loads of a chunk's own and its neighbours' records hit the CU's
+// cache after the first and gain nothing from it).  Buffer offsets are
+// 32-bit, so the sweeps take this path only while the state arrays are
+// below 2 GiB (wide_records).
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer resource: byte-addressed, no format
+constexpr int kLoadDevice = 16;            // the cache policy of ld_dev (device-coherent, sc1)
+__device__ __forceinline__ bool wide_records(int nq) { return (size_t)nq * 5 * sizeof(double) <= 0x7fffffffu; }
+
+template <int AUX, typename T>
+__device__ __forceinline__ void ld5w(const T* base, int nrec, int i, T (&v)[5]) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), 0, nrec * 5 * (int)sizeof(T), kBufferDword3);
+    const int off = i * 5 * (int)sizeof(T);
+    if constexpr (sizeof(T) == 8) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, AUX);
+        const auto c = __builtin_amdgcn_raw_buffer_load_b64(r, off + 32, 0, AUX);
+        const unsigned w[10] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], c[0], c[1]};
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            v[j] = __builtin_bit_cast(T, ((unsigned long long)w[2 * j + 1] << 32) | w[2 * j]);
+    } else {
+        // the words go through plain integers: a bit_cast straight from an
+        // element of the returned vector compiled to element 0 repeated
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX);
+        const unsigned b = __builtin_amdgcn_raw_buffer_load_b32(r, off + 16, 0, AUX);
+        const unsigned w[5] = {a[0], a[1], a[2], a[3], b};
+#pragma unroll
+        for (int j = 0; j < 5; ++j) v[j] = __builtin_bit_cast(T, w[j]);
+    }
+}
+// Record i of `base` (nrec records) as published by another wave, read
+// once.  The wide path is not an atomic: single reads only, never in a
+// wait loop (the compiler could keep its value across the loop's turns).
+template <typename T>
+__device__ __forceinline__ void read_published(bool wide, const T* base, int nrec, int i, T (&v)[5]) {
+    if (wide) {
+        ld5w<kLoadDevice>(base, nrec, i, v);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) v[j] = __builtin_bit_cast(T, ld_dev<T>(base + (size_t)i * 5 + j));
+    }
+}
+
 // True when another wave has already given up (checked every 4096 spins),
 // so that one expired wait ends the whole sweep quickly.
 __device__ __forceinline__ bool wait_failed(int* err, unsigned spins) {
@@ -101,6 +149,7 @@ __device__ __forceinline__ bool wait_failed(int* err, unsigned spins) {
 template <typename R>
 struct SweepArgs {
     int nC, nchunks;
+    int nq;                                 // records of the state arrays: interior and ghost cells
     int ntiles;
     const int* tile_chunk;                  // ntiles+1: first chunk of each tile (device order)
     const int* pos;                         // per chunk: its position in the flat order
@@ -232,20 +281,22 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
     R qnb[5] = {R(0), R(0), R(0), R(0), R(0)};
     R dv[5] = {R(0), R(0), R(0), R(0), R(0)};
     if (mine) load5r(A.q + (size_t)nb * 5, qnb);
-    if (rep_seen && pend) {  // the dependencies are most likely out: read them now
-        B b[5];
+    const bool wide = wide_records(A.nq);  // the same for every wave
+    // one read of every pending dependency (they are interior cells); a
+    // value still the sentinel leaves the lane pending
+    auto read_deps = [&] {
+        R b[5];
+        read_published(wide, static_cast<const R*>(poll), A.nC, nb, b);
         bool ok = true;
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            b[j] = ld_dev<R>(poll + (size_t)nb * 5 + j);
-            ok = ok && (b[j] != kSent);
-        }
+        for (int j = 0; j < 5; ++j) ok = ok && (__builtin_bit_cast(B, b[j]) != kSent);
         if (ok) {
 #pragma unroll
-            for (int j = 0; j < 5; ++j) dv[j] = __builtin_bit_cast(R, b[j]);
+            for (int j = 0; j < 5; ++j) dv[j] = b[j];
             pend = false;
         }
-    }
+    };
+    if (rep_seen && pend) read_deps();  // the dependencies are most likely out: read them now
     FacePre<R> pre;
     if (mine) {
         if constexpr (sizeof(R) == sizeof(double))
@@ -268,6 +319,9 @@ __device__ __forceinline__ bool sweep_chunk(const SweepArgs<R>& A, const FlowCon
             __builtin_amdgcn_s_sleep(2);
         }
     }
+    // wide levels: the chunk's latest dependency is out, so the others most
+    // likely are too; one read of each before the retry loop
+    if (!thin && pend) read_deps();
     // every lane reads its face's dependency, retrying the rare unpublished one
     while (__ballot(pend) != 0) {
         if (pend) {
