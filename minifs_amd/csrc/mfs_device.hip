@@ -32,10 +32,11 @@ void set_error(const std::string& msg);  // mfs_capi.cpp
 
 // The static scratch of mfs_step_noalloc: per-block partials of the
 // residual norms and time-step extrema for contexts of up to
-// kNoAllocMaxCells cells, then the two sweeps' ticket counters (2.03 MiB).
+// kNoAllocMaxCells cells, then the two sweeps' ticket counters and the
+// side-stream gradient's block counters (2.03 MiB).
 constexpr int kNoAllocMaxCells = 1 << 23;
 constexpr size_t kNoAllocPartDoubles = (size_t)(((kNoAllocMaxCells + 255) / 256 + 7) / 8 * 8) * 8;
-constexpr size_t kNoAllocTicketInts = 2 * 64 * 64;
+constexpr size_t kNoAllocTicketInts = 2 * 64 * 64 + 8 * 64;
 constexpr size_t kNoAllocBytes = kNoAllocPartDoubles * sizeof(double) + kNoAllocTicketInts * sizeof(int);
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -73,6 +74,15 @@ constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wa
 // sweep of 6M cells); 64 counters cost well under a tenth of a sweep.
 constexpr int kTicketGroups = 64;
 constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
+// A context's counters: the two sweeps' ticket sets, then the block
+// counters of the side-stream gradient (one per XCD).
+constexpr size_t kGradClaimOffset = 2 * (size_t)kTicketGroups * kTicketStride;
+constexpr size_t kTicketInts = kGradClaimOffset + (size_t)dev::kXcd * kTicketStride;
+// Wide launches hand out runs of 4 consecutive steps per ticket group: on
+// the 6.0M-cell mesh 1.2% faster per FP64 iteration than single steps
+// (1.0% mixed), and 8-step runs gained less.  The thin tail keeps single
+// steps: its levels are a few chunks each.
+constexpr int kWideRunShift = 2;
 constexpr unsigned kClampBlocks = 1024;  // grid of the ratio_dtmax clamp (grid-stride over the cells)
 // Where mfs_step runs CalculateGradient: 1 on the solver stream, fused with
 // the next iteration's residual pass; 2, 3, 4, 5 on a second stream,
@@ -250,6 +260,8 @@ using dev::kSweepWave;
 using dev::kSweepWavesPerCU;
 using dev::kTicketGroups;
 using dev::kTicketStride;
+using dev::kGradClaimOffset;
+using dev::kTicketInts;
 
 bool hip_ok(mfs_ctx* ctx, hipError_t e, const char* what) {
     if (e == hipSuccess) return true;
@@ -484,7 +496,7 @@ void forget_sweep_stream(int device, hipStream_t stream) {
 static_assert(kNoAllocPartDoubles ==
                   (size_t)(((kNoAllocMaxCells + dev::kBlock - 1) / dev::kBlock + dev::kXcd - 1) / dev::kXcd * dev::kXcd) * 8,
               "scratch partials: 8 doubles per cell block of the largest context");
-static_assert(kNoAllocTicketInts == 2 * (size_t)kTicketGroups * kTicketStride, "scratch tickets: both sweeps");
+static_assert(kNoAllocTicketInts == kTicketInts, "scratch tickets: both sweeps and the gradient's counters");
 static_assert(7 * (kNoAllocPartDoubles / 8) + 7 * (size_t)dev::kReduceBlock <= kNoAllocPartDoubles,
               "scratch partials: room for the reduction's folds after the largest context's records");
 
@@ -648,6 +660,11 @@ bool launch_diag(mfs_ctx* ctx, bool clamp_only) {
     return ctx->maxf == 4 ? launch_diag_t<4>(ctx, clamp_only) : launch_diag_t<6>(ctx, clamp_only);
 }
 
+// Whether the sweeps run as two launches each (wide levels, thin tail).
+bool sweep_split(const mfs_ctx* ctx) {
+    return !ctx->sweep_by_tiles && ctx->tail_grid > 0 && ctx->tail_u < ctx->nchunk;
+}
+
 // Workgroups of the thin-tail launch of a split sweep.
 int tail_workgroups(const mfs_ctx* ctx) {
     const int units = (ctx->nchunk - ctx->tail_u + ctx->sweep_waves - 1) / ctx->sweep_waves;
@@ -694,7 +711,7 @@ bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_t
     // A split sweep runs its wide levels on the resident grid and its thin
     // tail on a small one: the forward sweep ends with the tail and the
     // backward sweep starts with it.
-    const bool split = A.flat && ctx->tail_grid > 0 && ctx->tail_u < A.nchunks;
+    const bool split = sweep_split(ctx);
     const int nt = A.nchunks - ctx->tail_u;  // steps of the tail
     const dim3 tgrid((unsigned)(split ? tail_workgroups(ctx) : 1));
     // Each launch takes its tickets from one counter set and zeroes the
@@ -709,6 +726,7 @@ bool launch_sweeps_t(mfs_ctx* ctx, const std::function<bool()>& after, bool at_t
         A.ubeg = ubeg;
         A.uend = uend;
         A.ngroups = std::min(kTicketGroups, (int)g.x);
+        A.run_shift = g.x == grid.x ? dev::kWideRunShift : 0;
         A.tickets = tk[set];
         A.tickets_next = tk[1 - set];
         set ^= 1;
@@ -817,8 +835,17 @@ bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, h
                 sgeo = ctx->sgeo;
             else
                 sgeo = ctx->sgeo32;
-            hipLaunchKernelGGL(dev::k_grad_tet<R>, dim3(ctx->nblk_cell), dim3(kBlock), 0, st, ctx->nC, n,
-                               ctx->snb, sgeo, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz);
+            // on the second stream: one workgroup per CU (a multiple of 8,
+            // the same number on every XCD) claiming blocks; on the solver
+            // stream the full grid
+            const int gg = ((ctx->compute_units + dev::kXcd - 1) / dev::kXcd) * dev::kXcd;
+            if (st != ctx->stream && gg > 0 && gg < ctx->nblk_cell)
+                hipLaunchKernelGGL(dev::k_grad_tet_claim<R>, dim3(gg), dim3(kBlock), 0, st, ctx->nblk_cell, ctx->nC,
+                                   n, ctx->snb, sgeo, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz,
+                                   ctx->tickets + kGradClaimOffset);
+            else
+                hipLaunchKernelGGL(dev::k_grad_tet<R>, dim3(ctx->nblk_cell), dim3(kBlock), 0, st, ctx->nC, n,
+                                   ctx->snb, sgeo, ctx->snode, ctx->depmask, q, qn, ctx->vol, gx, gy, gz);
             if (!check_launch(ctx, "k_grad_tet")) return false;
         } else {
             hipLaunchKernelGGL((dev::k_grad<MAXF, R>), dim3(ctx->nblk_cell), dim3(kBlock), 0, st, ctx->nC,
@@ -1068,7 +1095,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
     }
     ok = ok && dalloc(ctx, ctx->part, partial_doubles((int)ctx->nblk_cell), "partials") && dalloc(ctx, ctx->run, 8, "run") &&
          dalloc(ctx, ctx->err, 4, "err") &&
-         dalloc(ctx, ctx->tickets, 2 * (size_t)kTicketGroups * kTicketStride, "sweep tickets") &&
+         dalloc(ctx, ctx->tickets, kTicketInts, "sweep tickets") &&
          dalloc(ctx, ctx->gface, (size_t)L.nB, "ghost faces");
     if (L.tri_slots) ok = ok && dalloc(ctx, ctx->snode, 12 * (size_t)L.nC, "slot nodes");
     ok = ok && dalloc(ctx, ctx->sdn, L.cfs.size(), "slot distances") && dalloc(ctx, ctx->sflip, L.nC, "slot sides");
@@ -1179,7 +1206,7 @@ extern "C" int mfs_ctx_create_ex(const mfs_mesh_view* mesh, const mfs_params* p,
         }
         ok = ok && hip_ok(ctx, hipMemsetAsync(ctx->err, 0, sizeof(int) * 4, ctx->stream), "memset err") &&
              hip_ok(ctx,
-                    hipMemsetAsync(ctx->tickets, 0, sizeof(int) * 2 * kTicketGroups * kTicketStride, ctx->stream),
+                    hipMemsetAsync(ctx->tickets, 0, sizeof(int) * kTicketInts, ctx->stream),
                     "memset tickets") &&
              hip_ok(ctx, hipMemsetAsync(ctx->dq, 0, sizeof(double) * 5 * n, ctx->stream), "memset dq") &&
              hip_ok(ctx, hipMemsetAsync(ctx->res, 0, sizeof(double) * 5 * L.nC, ctx->stream), "memset res") &&
@@ -1836,6 +1863,7 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
     A.ngroups = 0;
     A.ubeg = 0;
     A.uend = c->nchunk;
+    A.run_shift = 0;
     if constexpr (sizeof(R) == sizeof(double)) {
         E.ell_w = c->ell_w;
         E.qn = c->qn;

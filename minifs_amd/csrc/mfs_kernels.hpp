@@ -1037,6 +1037,52 @@ __global__ __launch_bounds__(kBlock) void k_grad_tet(int nC, int n, const int* _
                      gz);
 }
 
+// k_grad_tet on a small grid whose workgroups claim blocks of kBlock cells:
+// workgroup b takes the next unclaimed block of XCD x = b mod 8's span
+// (blocks x*span .. (x+1)*span-1, span = nblk / 8, the blocks xcd_block()
+// puts on that XCD) from counter x, until the span is done.  The next
+// block is claimed while the current one is worked on.  The last workgroup
+// to find its span done sets the counter back to zero for the next launch
+// (each of the G/8 workgroups of an XCD ends with exactly one claim past
+// the span, so the claim returning span + G/8 - 1 is the last).  nblk and
+// the grid G are multiples of 8; the counters are 64 ints apart and start
+// at zero.
+//
+// Launched on one workgroup per CU on the context's second stream, beside
+// the sweeps' thin tails (6.0M cells, reference order): the gradient takes
+// as long as on the full grid (23,440 workgroups), but its waves crowd the
+// tails' CUs less (FP64 iteration 3.63 -> 3.57 ms; mixed 3.04 -> 2.99).
+// Beside wide sweep levels (the shallower 2/128 order, no thin tail): FP64
+// 3.01 -> 2.95 ms, mixed 2.31 -> 2.28; a fixed share per workgroup instead
+// of claims left the late-starting workgroups' shares to the end there (the
+// gradient 1.40 -> 2.3 ms).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_grad_tet_claim(int nblk, int nC, int n, const int* __restrict__ snb,
+                                                           const typename Real<R>::vec4* __restrict__ sgeo,
+                                                           const int* __restrict__ snode,
+                                                           const int* __restrict__ depmask,
+                                                           const double* __restrict__ q, const R* __restrict__ qn,
+                                                           const double* __restrict__ vol, R* __restrict__ gx,
+                                                           R* __restrict__ gy, R* __restrict__ gz, int* counters) {
+    __shared__ int claimed;
+    const int x = (int)blockIdx.x % kXcd, per = (int)gridDim.x / kXcd, span = nblk / kXcd;
+    int* const ctr = counters + x * 64;
+    if (threadIdx.x == 0) claimed = atomicAdd(ctr, 1);
+    __syncthreads();
+    int k = claimed;
+    while (k < span) {
+        __syncthreads();  // every thread has read `claimed`
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1);
+        grad_tet_body<R>((x * span + k) * kBlock + (int)threadIdx.x, nC, n, snb, sgeo, snode, depmask, q, qn, vol,
+                         gx, gy, gz);
+        if (threadIdx.x == 0) claimed = next;
+        __syncthreads();
+        k = claimed;
+    }
+    if (threadIdx.x == 0 && k == span + per - 1) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The gradient of one iteration and the residual / time step / Diag pass of
 // the next, on tetrahedral meshes, in one launch.  One launch instead of
 // two lets each wave overlap the gradient's node gathers with the flux

@@ -19,10 +19,13 @@
 // cells of one (tile, level) group, so its cells never depend on one
 // another.  Two schedules, both on a grid sized to be resident at once:
 //   flat (default): the chunks in forward level order (`run`), the backward
-//     sweep walking it from the end; position u belongs to ticket group
-//     u mod ngroups, and a free wave of the group takes the group's next
-//     position from the group's counter, so waves that finish early take
-//     more chunks and no wave's delay holds back the chunks after it.
+//     sweep walking it from the end; the steps go to the ticket groups in
+//     runs of 2^run_shift consecutive steps, run k to group k mod ngroups,
+//     and a free wave of the group takes the group's next step from the
+//     group's counter, so waves that finish early take more chunks and no
+//     wave's delay holds back the chunks after it.  A group's workgroups
+//     share an XCD, so a run's chunks (neighbours in one tile and level,
+//     whose rows share cache lines) meet in one L2.
 //   tiles: a workgroup of W waves owns a tile, wave w taking the tile's
 //     chunks w, w+W, ... in level order; tiles go to workgroups
 //     round-robin, each workgroup walking its tiles in sweep order.
@@ -175,6 +178,9 @@ struct SweepArgs {
     // a sweep split in two launches runs its wide levels and its thin tail
     // apart, each launch ending when all its steps are done
     int ubeg, uend;
+    // flat schedule: the steps go to the ticket groups in runs of
+    // 2^run_shift consecutive steps (run k to group k mod ngroups)
+    int run_shift;
 };
 
 // Adds the LPC per-face products of this lane's cell in slot order (the
@@ -453,7 +459,8 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(SweepArgs<R> A, FlowConst
         if (lane == 0) tk = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tk = __shfl(tk, 0, 64);
         while (true) {
-            const int u = A.ubeg + grp + ng * tk;
+            const int sh = A.run_shift;
+            const int u = A.ubeg + ((((tk >> sh) * ng + grp) << sh) | (tk & ((1 << sh) - 1)));
             if (u >= A.uend) return;
             int next = 0;  // taken now, used after this chunk: its latency overlaps the chunk
             if (lane == 0) next = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
