@@ -36,7 +36,7 @@ void set_error(const std::string& msg);  // mfs_capi.cpp
 // side-stream gradient's block counters (2.03 MiB).
 constexpr int kNoAllocMaxCells = 1 << 23;
 constexpr size_t kNoAllocPartDoubles = (size_t)(((kNoAllocMaxCells + 255) / 256 + 7) / 8 * 8) * 8;
-constexpr size_t kNoAllocTicketInts = 2 * 64 * 64 + 8 * 64;
+constexpr size_t kNoAllocTicketInts = 2 * 64 * 64 + 2 * 8 * 64;
 constexpr size_t kNoAllocBytes = kNoAllocPartDoubles * sizeof(double) + kNoAllocTicketInts * sizeof(int);
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -75,9 +75,11 @@ constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wa
 constexpr int kTicketGroups = 64;
 constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
 // A context's counters: the two sweeps' ticket sets, then the block
-// counters of the side-stream gradient (one per XCD).
+// counters of the side-stream gradient and of the side-stream node values
+// (one per XCD each).
 constexpr size_t kGradClaimOffset = 2 * (size_t)kTicketGroups * kTicketStride;
-constexpr size_t kTicketInts = kGradClaimOffset + (size_t)dev::kXcd * kTicketStride;
+constexpr size_t kNodeClaimOffset = kGradClaimOffset + (size_t)dev::kXcd * kTicketStride;
+constexpr size_t kTicketInts = kNodeClaimOffset + (size_t)dev::kXcd * kTicketStride;
 // Wide launches hand out runs of 4 consecutive steps per ticket group: on
 // the 6.0M-cell mesh 1.2% faster per FP64 iteration than single steps
 // (1.0% mixed), and 8-step runs gained less.  The thin tail keeps single
@@ -261,6 +263,7 @@ using dev::kSweepWavesPerCU;
 using dev::kTicketGroups;
 using dev::kTicketStride;
 using dev::kGradClaimOffset;
+using dev::kNodeClaimOffset;
 using dev::kTicketInts;
 
 bool hip_ok(mfs_ctx* ctx, hipError_t e, const char* what) {
@@ -804,8 +807,15 @@ bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, h
     }
     {
         Timed t(ctx, "node", st);
-        hipLaunchKernelGGL(dev::k_node<R>, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, st, ctx->nN, ctx->nlo,
-                           ctx->ell_off, ctx->ell_ab, ew, ctx->nden, q, qn);
+        // on the second stream: one workgroup per CU claiming blocks, as the gradient
+        const int nbn = (int)nblk_xcd(ctx->nN);
+        const int gn = ((ctx->compute_units + dev::kXcd - 1) / dev::kXcd) * dev::kXcd;
+        if (st != ctx->stream && gn > 0 && gn < nbn)
+            hipLaunchKernelGGL(dev::k_node_claim<R>, dim3(gn), dim3(kBlock), 0, st, nbn, ctx->nN, ctx->nlo,
+                               ctx->ell_off, ctx->ell_ab, ew, ctx->nden, q, qn, ctx->tickets + kNodeClaimOffset);
+        else
+            hipLaunchKernelGGL(dev::k_node<R>, dim3(nbn), dim3(kBlock), 0, st, ctx->nN, ctx->nlo, ctx->ell_off,
+                               ctx->ell_ab, ew, ctx->nden, q, qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
             hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, st, ctx->nsym,
