@@ -131,6 +131,72 @@ __global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restri
                            S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
 }
 
+// The node values and the gradient of the batch on a small grid whose
+// workgroups claim blocks (on the second stream, beside the backward
+// sweep), as k_node_claim and k_grad_tet_claim do for one solve: workgroup
+// b takes the next unclaimed global block b mod 8 + 8k from counter
+// b mod 8, so an XCD walks each solve's blocks of that XCD in order.  The
+// block counts are multiples of 8 (each solve owns whole groups of 8); the
+// counters are 64 ints apart, zero before the launch and left at zero.  64
+// seeded solves, mixed precision, one workgroup per CU: 11.52 -> 11.34 ms
+// per batch iteration (two per CU: no gain; fixed shares per workgroup:
+// 11.46).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_b_node_claim(const BatchSolve<R>* __restrict__ T,
+                                                         const int* __restrict__ node_solve, int nblk, int* counters) {
+    __shared__ int claimed;
+    const int x = (int)blockIdx.x % kXcd, per = (int)gridDim.x / kXcd, span = nblk / kXcd;
+    int* const ctr = counters + x * 64;
+    if (threadIdx.x == 0) claimed = atomicAdd(ctr, 1);
+    __syncthreads();
+    int k = claimed;
+    while (k < span) {
+        __syncthreads();  // every thread has read `claimed`
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1);
+        const int gb = k * kXcd + x;
+        const BatchSolve<R>& S = T[node_solve[gb]];
+        const int lb = gb - S.blk0_node;
+        const int lg = (lb % kXcd) * (S.nblk_node / kXcd) + lb / kXcd;
+        node_body<R>(lg * kBlock + (int)threadIdx.x, lg * (kBlock / 64) + (int)threadIdx.x / 64, S.nN, S.nlo, S.ell_off,
+                     S.ell_ab, S.ell_w, S.nden, S.qnew, S.qn);
+        if (threadIdx.x == 0) claimed = next;
+        __syncthreads();
+        k = claimed;
+    }
+    if (threadIdx.x == 0 && k == span + per - 1) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int MAXF, typename R>
+__global__ __launch_bounds__(kBlock) void k_b_grad_claim(const BatchSolve<R>* __restrict__ T,
+                                                         const int* __restrict__ cell_solve, int nblk, int* counters) {
+    __shared__ int claimed;
+    const int x = (int)blockIdx.x % kXcd, per = (int)gridDim.x / kXcd, span = nblk / kXcd;
+    int* const ctr = counters + x * 64;
+    if (threadIdx.x == 0) claimed = atomicAdd(ctr, 1);
+    __syncthreads();
+    int k = claimed;
+    while (k < span) {
+        __syncthreads();  // every thread has read `claimed`
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1);
+        const int gb = k * kXcd + x;
+        const BatchSolve<R>& S = T[cell_solve[gb]];
+        const int lb = gb - S.blk0_cell;
+        const int lg = (lb % kXcd) * (S.nblk_cell / kXcd) + lb / kXcd;
+        if (MAXF == 4 && S.snode)
+            grad_tet_body<R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.sw.snb, S.sw.sgeo, S.snode, S.sw.depmask,
+                             S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
+        else
+            grad_body<MAXF, R>(lg * kBlock + (int)threadIdx.x, S.nC, S.n, S.cfo, S.cf, S.fcell, S.fgeo_r, S.fno, S.fn,
+                               S.btype, S.qnew, S.qn, S.vol, S.gx, S.gy, S.gz);
+        if (threadIdx.x == 0) claimed = next;
+        __syncthreads();
+        k = claimed;
+    }
+    if (threadIdx.x == 0 && k == span + per - 1) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The gradient of iteration i (table T) fused with the residual / time-step
 // pass of iteration i+1 (table T1), as in the single-solve step: the same
 // solves in both tables, the gradient feeding nothing of iteration i+1.

@@ -1723,7 +1723,10 @@ struct mfs_batch {
     hipEvent_t table_copied = nullptr;
     bool table_pending = false;
     int sweep_grid = 0;
-    int* tickets = nullptr;       // ticket counters of the batch sweeps: forward, then backward
+    int compute_units = 0;
+    // ticket counters of the batch sweeps (forward, then backward), then the
+    // block counters of the second stream's node values and gradient
+    int* tickets = nullptr;
     // gradient_stream 2 (after the update) / 4 (after the next forward
     // sweep): the node values and gradient of a batch iteration on gstream;
     // gdone[k & 1] after iteration k's, which reads the state buffers that
@@ -1907,14 +1910,26 @@ void fill_entry(dev::BatchSolve<R>& E, mfs_ctx* c, int cur, int blk0_cell, int b
 template <int MAXF, typename R>
 bool batch_gradient(mfs_batch* b, const dev::BatchSolve<R>* T, hipStream_t st) {
     constexpr bool MIXED = sizeof(R) == sizeof(float);
-    hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, st, T, b->node_solve);
+    // on the second stream: one workgroup per CU (a multiple of 8) claiming blocks
+    const int gw = ((b->compute_units + dev::kXcd - 1) / dev::kXcd) * dev::kXcd;
+    int* const claim = b->tickets + kGradClaimOffset;
+    const bool side = st != b->stream && gw > 0;
+    if (side && gw < b->nnode_blk)
+        hipLaunchKernelGGL((dev::k_b_node_claim<R>), dim3(gw), dim3(kBlock), 0, st, T, b->node_solve, b->nnode_blk,
+                           claim);
+    else
+        hipLaunchKernelGGL((dev::k_b_node<R>), dim3(b->nnode_blk), dim3(kBlock), 0, st, T, b->node_solve);
     for (size_t a = 0; a < b->active.size(); ++a) {
         mfs_ctx* c = b->slots[b->active[a]];
         if (c->nsym > 0)
             hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(c->nsym)), dim3(kBlock), 0, st, c->nsym, c->sym_node,
                                c->sym_n, MIXED ? (R*)c->qn32 : (R*)c->qn);
     }
-    hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, st, T, b->cell_solve);
+    if (side && gw < b->ncell_blk)
+        hipLaunchKernelGGL((dev::k_b_grad_claim<MAXF, R>), dim3(gw), dim3(kBlock), 0, st, T, b->cell_solve,
+                           b->ncell_blk, claim + (kNodeClaimOffset - kGradClaimOffset));
+    else
+        hipLaunchKernelGGL((dev::k_b_grad<MAXF, R>), dim3(b->ncell_blk), dim3(kBlock), 0, st, T, b->cell_solve);
     return batch_ok(b, hipGetLastError(), "batch gradient");
 }
 
@@ -2116,7 +2131,7 @@ extern "C" int mfs_batch_create_ex(int precision, const mfs_batch_options* opt, 
         delete b;
         return MFS_ERR_HIP;
     }
-    const size_t tbytes = sizeof(int) * 2 * kTicketGroups * kTicketStride;
+    const size_t tbytes = sizeof(int) * kTicketInts;
     bool ok = batch_ok(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") &&
               batch_ok(b, hipEventCreateWithFlags(&b->table_copied, hipEventDisableTiming), "hipEventCreate") &&
               batch_ok(b, hipMalloc(&b->tickets, tbytes), "hipMalloc(batch tickets)") &&
@@ -2175,6 +2190,7 @@ extern "C" int mfs_batch_attach(mfs_batch* b, mfs_ctx* c, int* slot) {
         b->maxf = c->maxf;
         b->lpc = c->lpc;
         b->sweep_grid = std::max(1, std::min(per_cu, kSweepWavesPerCU / kBatchSweepWaves) * cus);
+        b->compute_units = cus;
     }
     // the solve's work goes to the batch stream from now on
     if (!batch_ok(b, hipStreamSynchronize(c->stream), "mfs_batch_attach: context stream")) return MFS_ERR_HIP;
