@@ -208,7 +208,7 @@ int mfs_step(mfs_ctx* ctx, int n);
  * graph capture, serving loops that forbid hipMalloc); it has no
  * counterpart in the reference.  It calls no hipMalloc, hipFree or other
  * allocating routine: the reduction partials, sweep ticket counters and
- * the side-stream node and gradient kernels' block counters live in a
+ * the block counters of the second stream's claiming kernels live in a
  * static device array of the library (2.03 MiB per GPU, held until the
  * process ends), and the n history records (7 doubles each, as
  * mfs_get_history gives them) go to hist_dev, a device buffer of 7*n

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_b_grad(const BatchSolve<R>* __restri
 
 // The node values and the gradient of the batch on a small grid whose
 // workgroups claim blocks (on the second stream, beside the backward
-// sweep), as k_node_claim and k_grad_tet_claim do for one solve: workgroup
+// sweep), as k_grad_tet_claim does for one solve's gradient: workgroup
 // b takes the next unclaimed global block b mod 8 + 8k from counter
 // b mod 8, so an XCD walks each solve's blocks of that XCD in order.  The
 // block counts are multiples of 8 (each solve owns whole groups of 8); the

@@ -74,9 +74,9 @@ constexpr unsigned kSpinLimit = 1u << 23;  // bound on the waits of one sweep wa
 // sweep of 6M cells); 64 counters cost well under a tenth of a sweep.
 constexpr int kTicketGroups = 64;
 constexpr int kTicketStride = 64;  // ints between counters (256 bytes)
-// A context's counters: the two sweeps' ticket sets, then the block
-// counters of the side-stream gradient and of the side-stream node values
-// (one per XCD each).
+// The counters of a context or a batch: the two sweeps' ticket sets, then
+// one block counter per XCD for each of two claiming kernels on the second
+// stream (a context's gradient; a batch's node values and gradient).
 constexpr size_t kGradClaimOffset = 2 * (size_t)kTicketGroups * kTicketStride;
 constexpr size_t kNodeClaimOffset = kGradClaimOffset + (size_t)dev::kXcd * kTicketStride;
 constexpr size_t kTicketInts = kNodeClaimOffset + (size_t)dev::kXcd * kTicketStride;
@@ -807,15 +807,8 @@ bool launch_gradient_t(mfs_ctx* ctx, bool next = false, double next_cfl = 0.0, h
     }
     {
         Timed t(ctx, "node", st);
-        // on the second stream: one workgroup per CU claiming blocks, as the gradient
-        const int nbn = (int)nblk_xcd(ctx->nN);
-        const int gn = ((ctx->compute_units + dev::kXcd - 1) / dev::kXcd) * dev::kXcd;
-        if (st != ctx->stream && gn > 0 && gn < nbn)
-            hipLaunchKernelGGL(dev::k_node_claim<R>, dim3(gn), dim3(kBlock), 0, st, nbn, ctx->nN, ctx->nlo,
-                               ctx->ell_off, ctx->ell_ab, ew, ctx->nden, q, qn, ctx->tickets + kNodeClaimOffset);
-        else
-            hipLaunchKernelGGL(dev::k_node<R>, dim3(nbn), dim3(kBlock), 0, st, ctx->nN, ctx->nlo, ctx->ell_off,
-                               ctx->ell_ab, ew, ctx->nden, q, qn);
+        hipLaunchKernelGGL(dev::k_node<R>, dim3(nblk_xcd(ctx->nN)), dim3(kBlock), 0, st, ctx->nN, ctx->nlo,
+                           ctx->ell_off, ctx->ell_ab, ew, ctx->nden, q, qn);
         if (!check_launch(ctx, "k_node")) return false;
         if (ctx->nsym > 0) {
             hipLaunchKernelGGL(dev::k_node_symm<R>, dim3(nblk(ctx->nsym)), dim3(kBlock), 0, st, ctx->nsym,

@@ -840,37 +840,6 @@ __global__ __launch_bounds__(kBlock) void k_node(int nN, const int* __restrict__
                  ell_w, nden, q, qn);
 }
 
-// k_node on a small grid whose workgroups claim blocks of kBlock nodes, as
-// k_grad_tet_claim does (counters at `counters`, 64 ints apart, zero before
-// the launch and left at zero).  On the second stream beside the thin
-// tails (6.0M cells) one workgroup per CU: FP64 iteration 3.60 -> 3.56 ms,
-// mixed unchanged, the 2/128 order unchanged in FP64 and 1% faster mixed.
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_node_claim(int nblk, int nN, const int* __restrict__ nlo,
-                                                       const int* __restrict__ ell_off, const int2* __restrict__ ell_ab,
-                                                       const R* __restrict__ ell_w, const double* __restrict__ nden,
-                                                       const double* __restrict__ q, R* __restrict__ qn,
-                                                       int* counters) {
-    __shared__ int claimed;
-    const int x = (int)blockIdx.x % kXcd, per = (int)gridDim.x / kXcd, span = nblk / kXcd;
-    int* const ctr = counters + x * 64;
-    if (threadIdx.x == 0) claimed = atomicAdd(ctr, 1);
-    __syncthreads();
-    int k = claimed;
-    while (k < span) {
-        __syncthreads();  // every thread has read `claimed`
-        int next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(ctr, 1);
-        const int lb = x * span + k;
-        node_body<R>(lb * kBlock + (int)threadIdx.x, lb * (kBlock / 64) + (int)threadIdx.x / 64, nN, nlo, ell_off,
-                     ell_ab, ell_w, nden, q, qn);
-        if (threadIdx.x == 0) claimed = next;
-        __syncthreads();
-        k = claimed;
-    }
-    if (threadIdx.x == 0 && k == span + per - 1) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Symmetry-plane velocity projection of the node values (cal_gradient.cpp:2332-2360).
 template <typename R>
 __global__ void k_node_symm(int nsym, const int* __restrict__ sym_node, const double* __restrict__ sym_n,
